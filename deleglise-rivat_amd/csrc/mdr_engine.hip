@@ -1,0 +1,843 @@
+// Host engine and C ABI of the MI355X Deleglise-Rivat Mertens engine.
+//
+// One context = one GPU, one stream, cached device buffers.  A call plans the
+// parameters of every X (reference rules or the tuned GPU split), builds or
+// reuses the table, partitions the quotient phase into chunks and runs the
+// persistent kernel; results come back as 128-bit sums per X.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mertens_hip.h"
+#include "mdr_device.hpp"
+#include "mdr_kernels.hpp"
+#include "mdr_tune.hpp"
+
+using namespace mdr;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+i64 isqrt_host(i64 n) {
+    if (n <= 0) return 0;
+    i64 s = (i64)std::sqrt((double)n);
+    while ((__int128)(s + 1) * (s + 1) <= n) ++s;
+    while ((__int128)s * s > n) --s;
+    return s;
+}
+
+}  // namespace
+
+struct mdr_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int ncu = 0;
+    int quot_blocks_per_cu = 1;
+    std::mutex mu;
+    std::string err;
+    hipEvent_t ev[8] = {};
+
+    // table
+    bool tab_valid = false;
+    i64 tU = 0, tud = 0, tmu = 0, tW = 0, tnseg = 0, tmusegs = 0;
+    DevBuf Md, cpl, cck, mu6, segsum, scan_tmp;
+    // sieving primes
+    i64 primes_root = -1;
+    int n_w = 0, n_small = 0, n_sq = 0, n_sq_small = 0;
+    DevBuf stencil, wprimes, wrcp, wweights, sqprimes;
+    // work
+    DevBuf xparam, counts, recs, out128, ctr, stamps, comp_out;
+    i64 last_nrec = 0;
+    std::vector<uint32_t> last_stamps;
+    std::vector<ChunkRec> last_recs;
+    std::vector<XParam> last_xp;
+
+    TableParam tp() const {
+        TableParam t;
+        t.U = tU;
+        t.ud = tud;
+        t.W = tW;
+        t.mu_lim = tmu;
+        t.Md = Md.as<int16_t>();
+        t.cpl = cpl.as<u64>();
+        t.cck = cck.as<int32_t>();
+        t.mu6 = mu6.as<uint32_t>();
+        return t;
+    }
+};
+
+namespace {
+
+mdr_status fail(mdr_ctx* c, mdr_status code, const std::string& m) {
+    c->err = m;
+    return code;
+}
+mdr_status hip_fail(mdr_ctx* c, hipError_t e, const char* what) {
+    const mdr_status code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MDR_ERR_NOMEM : MDR_ERR_HIP;
+    return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPT(call, what)                                  \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hip_fail(c, e_, what); \
+    } while (0)
+
+// ---- parameters -------------------------------------------------------------
+// Reference rules: mertens_dr.hpp:914-923 (cx), :931-962 (dense cutoff),
+// :1401-1419 (streaming cutoff), :1451-1461 (dense tier and mu limit),
+// :1027-1041 and :1421-1435 (strategy).
+double ref_cx(i64 X, const mdr_opts& o) {
+    if (o.cx > 0.1 && o.cx < 5.0) return o.cx;
+    if (X >= 1000000000000000LL) return 1.35;
+    if (X >= 10000000000000LL) return 1.15;
+    if (X >= 100000000000LL) return 0.95;
+    return 0.70;
+}
+
+i64 ref_dense_u(i64 X, const mdr_opts& o) {
+    if (X <= 50000000LL) return X;
+    const double ll = std::log(std::max(2.0, std::log((double)X)));
+    const double fx = (o.fx >= 0.1 && o.fx <= 3.0) ? o.fx : 0.32;
+    i64 u = (i64)(fx * std::pow((double)X / ll, 2.0 / 3.0));
+    const i64 S = isqrt_host(X);
+    if (u < 3 * S) u = 3 * S;
+    const i64 cap = (o.u_cap_dense >= 100000000LL && o.u_cap_dense <= 4000000000LL) ? o.u_cap_dense : 1500000000LL;
+    return std::min(u, cap);
+}
+
+i64 ref_stream_u(i64 X, const mdr_opts& o, double cx) {
+    if (X < 50000000LL) return X;
+    const double ll = std::log(std::max(2.0, std::log((double)X)));
+    double fx = 0.95;
+    if (X >= 1000000000000000LL) fx = 1.25;
+    if (X >= 10000000000000000LL) fx = 1.50;
+    i64 u = (i64)(fx * std::pow((double)X / ll, 2.0 / 3.0));
+    const i64 S = isqrt_host(X);
+    if (u < 3 * S) u = 3 * S;
+    u = ((u + 255) / 256) * 256;
+    i64 cap = (o.u_cap >= 100000000LL && o.u_cap <= 10000000000LL) ? o.u_cap : 8000000000LL;
+    const i64 ends = (i64)(cx * std::sqrt((double)X)) + 1000;
+    cap = std::max(cap, std::max(S + 1000, ends));
+    return std::min(u, cap);
+}
+
+struct XPlan {
+    i64 X = 0;
+    int kind = 0;      // 0 constant, 1 table lookup, 2 Deleglise-Rivat sum
+    i64 value = 0;     // kind 0
+    bool streaming = false;
+    i64 u = 0, ud = 0, mu_lim = 0, N = 0;
+    double cx = 0;
+};
+
+mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
+    if (o.strategy < 0 || o.strategy > 3) return fail(c, MDR_ERR_INVALID, "strategy out of range");
+    if (o.cx != 0 && !(o.cx > 0.1 && o.cx < 5.0)) return fail(c, MDR_ERR_INVALID, "cx outside (0.1, 5.0)");
+    if (o.u_cap != 0 && (o.u_cap < 100000000LL || o.u_cap > 10000000000LL))
+        return fail(c, MDR_ERR_INVALID, "u_cap outside [1e8, 1e10]");
+    if (o.fx != 0 && !(o.fx >= 0.1 && o.fx <= 3.0)) return fail(c, MDR_ERR_INVALID, "fx outside [0.1, 3.0]");
+    if (o.u_cap_dense != 0 && (o.u_cap_dense < 100000000LL || o.u_cap_dense > 4000000000LL))
+        return fail(c, MDR_ERR_INVALID, "u_cap_dense outside [1e8, 4e9]");
+    if (o.partition < 0 || o.partition > 1) return fail(c, MDR_ERR_INVALID, "partition out of range");
+    if (o.params < 0 || o.params > 1) return fail(c, MDR_ERR_INVALID, "params out of range");
+    if (o.sched < 0 || o.sched > 1) return fail(c, MDR_ERR_INVALID, "sched out of range");
+    if (o.chunk < 0) return fail(c, MDR_ERR_INVALID, "chunk must be >= 0");
+    return MDR_OK;
+}
+
+XPlan plan_x(i64 X, const mdr_opts& o) {
+    XPlan p;
+    p.X = X;
+    if (X < 1) { p.kind = 0; p.value = 0; return p; }
+    if (X <= 3) { p.kind = 0; p.value = X == 1 ? 1 : X == 2 ? 0 : -1; return p; }
+    const double cx = ref_cx(X, o);
+    bool streaming;
+    if (o.strategy == MDR_STRATEGY_FORCE_STREAMING) streaming = X > 100000LL;
+    else if (o.strategy == MDR_STRATEGY_STREAMING) streaming = X >= 100000000000LL;
+    else if (o.strategy == MDR_STRATEGY_DENSE) streaming = false;
+    else streaming = X >= 100000000000LL && !o.fx_set && !o.u_cap_dense_set;
+    if (X <= 50000000LL && !(streaming && o.strategy == MDR_STRATEGY_FORCE_STREAMING)) {
+        p.kind = 1;   // sieve to X and read M(X)
+        p.u = p.ud = p.mu_lim = X;
+        return p;
+    }
+    p.streaming = streaming;
+    p.cx = cx;
+    const i64 A_max = (i64)(cx * std::sqrt((double)X)) + 1000;
+    if (o.params == MDR_PARAMS_TUNED && X >= 100000000LL) {
+        const TuneChoice t = tune_lookup(X, streaming);
+        p.cx = t.cx;
+        const i64 A2 = (i64)(p.cx * std::sqrt((double)X)) + 1000;
+        p.u = t.u;
+        p.ud = streaming ? std::min(p.u, std::max<i64>(t.ud, ((A2 + 63) / 64) * 64)) : p.u;
+        p.N = X / p.u;
+        p.mu_lim = std::max(A2, p.N + 1000);
+    } else if (streaming) {
+        p.u = ref_stream_u(X, o, cx);
+        const i64 ends = (A_max + 255) & ~255LL;
+        const i64 target = std::min<i64>(128000000LL, (p.u / 4 + 255) & ~255LL);
+        p.ud = std::min(p.u, std::max(target, ends));
+        p.N = X / p.u;
+        p.mu_lim = std::max(A_max, p.N + 1000);
+    } else {
+        p.u = ref_dense_u(X, o);
+        p.ud = p.u;
+        p.N = X / p.u;
+        p.mu_lim = std::max(A_max, p.N + 1000);
+    }
+    p.kind = X <= p.u ? 1 : 2;
+    return p;
+}
+
+// ---- sieving primes and stencil ------------------------------------------------
+mdr_status ensure_primes(mdr_ctx* c, i64 U) {
+    const i64 root = isqrt_host(U) + 1;
+    if (root <= c->primes_root && c->stencil.p) return MDR_OK;
+    std::vector<uint8_t> comp((size_t)root + 1, 0);
+    std::vector<uint32_t> wp, sq;
+    std::vector<double> wr;
+    std::vector<uint8_t> ww;
+    for (i64 i = 2; i <= root; ++i) {
+        if (comp[(size_t)i]) continue;
+        for (i64 j = i * i; j <= root; j += i) comp[(size_t)j] = 1;
+        if (i >= 13) {
+            wp.push_back((uint32_t)i);
+            wr.push_back(1.0 / (double)i);
+            const unsigned bw = 64u - (unsigned)__builtin_clzll((unsigned long long)(i - 1));
+            ww.push_back((uint8_t)(bw | 1u));
+        }
+        if (i >= 7) sq.push_back((uint32_t)i);
+    }
+    c->n_w = (int)wp.size();
+    c->n_small = (int)(std::lower_bound(wp.begin(), wp.end(), 256u) - wp.begin());
+    c->n_sq = (int)sq.size();
+    c->n_sq_small = (int)(std::lower_bound(sq.begin(), sq.end(), 222u) - sq.begin());
+    HIPT(c->wprimes.ensure(wp.size() * 4 + 4), "alloc primes");
+    HIPT(c->wrcp.ensure(wr.size() * 8 + 8), "alloc primes");
+    HIPT(c->wweights.ensure(ww.size() + 4), "alloc primes");
+    HIPT(c->sqprimes.ensure(sq.size() * 4 + 4), "alloc primes");
+    if (!wp.empty()) {
+        HIPT(hipMemcpy(c->wprimes.p, wp.data(), wp.size() * 4, hipMemcpyHostToDevice), "upload primes");
+        HIPT(hipMemcpy(c->wrcp.p, wr.data(), wr.size() * 8, hipMemcpyHostToDevice), "upload primes");
+        HIPT(hipMemcpy(c->wweights.p, ww.data(), ww.size(), hipMemcpyHostToDevice), "upload primes");
+    }
+    if (!sq.empty()) HIPT(hipMemcpy(c->sqprimes.p, sq.data(), sq.size() * 4, hipMemcpyHostToDevice), "upload primes");
+    if (!c->stencil.p) {
+        // value for n with n = x+1 (mod period): weights of 2,3,5,7,11 and the
+        // square flags of 4, 9 and 25 (mertens_dr.hpp:784-793 uses 2..11, 4, 9).
+        std::vector<uint8_t> st((size_t)STENCIL_PERIOD + SEG);
+        for (size_t x = 0; x < st.size(); ++x) {
+            const unsigned v = (unsigned)(x % STENCIL_PERIOD) + 1;
+            unsigned a = 0;
+            for (unsigned p : {2u, 3u, 5u, 7u, 11u})
+                if (v % p == 0) a += (32u - (unsigned)__builtin_clz(p - 1)) | 1u;
+            if (v % 4 == 0 || v % 9 == 0 || v % 25 == 0) a |= 0x80u;
+            st[x] = (uint8_t)a;
+        }
+        HIPT(c->stencil.ensure(st.size()), "alloc stencil");
+        HIPT(hipMemcpy(c->stencil.p, st.data(), st.size(), hipMemcpyHostToDevice), "upload stencil");
+    }
+    c->primes_root = root;
+    return MDR_OK;
+}
+
+// ---- table -----------------------------------------------------------------------
+mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats) {
+    if (U < 1) U = 1;
+    if (ud > U) ud = U;
+    if (ud < U && (ud % 64) != 0) return fail(c, MDR_ERR_INVALID, "u_dense must be a multiple of 64 below U");
+    if (mu_lim > U) return fail(c, MDR_ERR_INVALID, "mu_limit above U");
+    if (mu_lim < 1) mu_lim = 1;
+    if (c->tab_valid && c->tU == U && c->tud == ud && c->tmu == mu_lim) {
+        if (stats) stats->table_reused = 1;
+        return MDR_OK;
+    }
+    c->tab_valid = false;
+    mdr_status s = ensure_primes(c, U);
+    if (s != MDR_OK) return s;
+    const i64 nseg = (U + SEG - 1) / SEG;
+    const i64 musegs = (mu_lim + SEG - 1) / SEG;
+    const i64 W = U > ud ? (U - ud + 63) / 64 : 0;
+    HIPT(c->Md.ensure((size_t)(ud + 1) * 2), "alloc dense table");
+    HIPT(c->cpl.ensure((size_t)W * 16), "alloc planes");
+    HIPT(c->cck.ensure((size_t)W * 4), "alloc checkpoints");
+    HIPT(c->mu6.ensure((size_t)musegs * (SEG / 48) * 4), "alloc mu");
+    HIPT(c->segsum.ensure((size_t)nseg * 8), "alloc segment sums");
+    HIPT(c->scan_tmp.ensure((size_t)scan_tmp_len(nseg) * 8), "alloc scan");
+    HIPT(hipMemsetAsync(c->Md.p, 0, 2, c->st), "clear");
+    HIPT(hipEventRecord(c->ev[0], c->st), "event");
+    SieveArgs a;
+    a.U = U;
+    a.ud = ud;
+    a.W = W;
+    a.mu_segs = musegs;
+    a.stencil = c->stencil.as<uint8_t>();
+    a.wprimes = c->wprimes.as<uint32_t>();
+    a.wrcp = c->wrcp.as<double>();
+    a.wweights = c->wweights.as<uint8_t>();
+    a.n_w = c->n_w;
+    a.n_small = c->n_small;
+    a.sqprimes = c->sqprimes.as<uint32_t>();
+    a.n_sq = c->n_sq;
+    a.n_sq_small = c->n_sq_small;
+    a.Md = c->Md.as<int16_t>();
+    a.cpl = c->cpl.as<u64>();
+    a.cck = c->cck.as<int32_t>();
+    a.mu6 = c->mu6.as<uint32_t>();
+    a.segsum = c->segsum.as<i64>();
+    const i64 maxgrid = 1 << 30;
+    for (i64 s0 = 0; s0 < nseg; s0 += maxgrid) {
+        a.seg_first = s0;
+        const i64 nb = std::min(maxgrid, nseg - s0);
+        hipLaunchKernelGGL(k_sieve, dim3((unsigned)nb), dim3(SIEVE_THREADS), 0, c->st, a);
+    }
+    HIPT(hipGetLastError(), "launch sieve");
+    HIPT(scan_exclusive_i64(c->segsum.as<i64>(), c->segsum.as<i64>(), nseg, c->scan_tmp.as<i64>(),
+                            scan_tmp_len(nseg), nullptr, c->st),
+         "scan segments");
+    FixupArgs f;
+    f.ud = ud;
+    f.W = W;
+    f.Md = c->Md.as<int16_t>();
+    f.cck = c->cck.as<int32_t>();
+    f.segoff = c->segsum.as<i64>();
+    const i64 nfix = std::max(ud, W);
+    if (nfix > 0) hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)((nfix + 255) / 256)), dim3(256), 0, c->st, f);
+    HIPT(hipGetLastError(), "launch fixup");
+    HIPT(hipEventRecord(c->ev[1], c->st), "event");
+    HIPT(hipEventSynchronize(c->ev[1]), "table build");
+    c->tU = U;
+    c->tud = ud;
+    c->tmu = mu_lim;
+    c->tW = W;
+    c->tnseg = nseg;
+    c->tmusegs = musegs;
+    c->tab_valid = true;
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+        stats->table_ms += ms;
+        stats->table_bytes += 2.0 * (double)(ud + 1) + 20.0 * (double)W + (double)musegs * (SEG / 12);
+    }
+    return MDR_OK;
+}
+
+__global__ void k_lookup(TableParam tp, const i64* qs, const int* xi, int n, u64* out128) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const i64 v = get_M(tp, qs[i]);
+    out128[2 * xi[i]] = (u64)v;
+    out128[2 * xi[i] + 1] = (u64)(v >> 63);
+}
+
+// ---- quotient phase ------------------------------------------------------------------
+mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts& o, long long* out,
+                     mdr_stats* stats) {
+    const size_t nx = plans.size();
+    // table covering every X
+    i64 U = 1, mu_lim = 1, ud_stream = 0;
+    bool any_stream = false;
+    for (const XPlan& p : plans) {
+        if (p.kind == 0) continue;
+        U = std::max(U, std::max(p.u, p.mu_lim));
+        if (p.kind == 1) U = std::max(U, p.X);
+        mu_lim = std::max(mu_lim, p.mu_lim);
+        if (p.kind == 2 && p.streaming) {
+            any_stream = true;
+            ud_stream = std::max(ud_stream, p.ud);
+        }
+    }
+    i64 ud = any_stream ? std::min(U, ud_stream) : U;
+    if (ud < U) ud = (ud / 64) * 64;
+    const auto t_call = std::chrono::steady_clock::now();
+    bool need_table = false;
+    for (const XPlan& p : plans) need_table |= p.kind != 0;
+    if (need_table) {
+        mdr_status s = build_table(c, U, ud, mu_lim, stats);
+        if (s != MDR_OK) return s;
+    }
+    const TableParam tp = c->tp();
+
+    std::vector<XParam> xps;
+    std::vector<int> xidx;
+    std::vector<i64> lq;
+    std::vector<int> lx;
+    i64 slots = 0;
+    for (size_t i = 0; i < nx; ++i) {
+        const XPlan& p = plans[i];
+        if (p.kind == 1) { lq.push_back(p.X); lx.push_back((int)i); }
+        if (p.kind != 2) continue;
+        XParam xp{};
+        xp.X = p.X;
+        xp.u = p.u;
+        xp.N = p.N;
+        xp.T = count_coprime6(p.N);
+        xp.slot_base = slots;
+        xp.cx = p.cx;
+        xp.n6 = p.N / 6;
+        xp.n3 = p.N / 3;
+        xp.n2 = p.N / 2;
+        slots += 2 * xp.T;
+        xps.push_back(xp);
+        xidx.push_back((int)i);
+    }
+    HIPT(c->out128.ensure(nx * 16 + 16), "alloc results");
+    HIPT(hipMemsetAsync(c->out128.p, 0, nx * 16, c->st), "clear results");
+    if (!lq.empty()) {
+        DevBuf& b = c->comp_out;
+        HIPT(b.ensure(lq.size() * 12 + 16), "alloc lookups");
+        HIPT(hipMemcpyAsync(b.p, lq.data(), lq.size() * 8, hipMemcpyHostToDevice, c->st), "upload");
+        HIPT(hipMemcpyAsync((char*)b.p + lq.size() * 8, lx.data(), lx.size() * 4, hipMemcpyHostToDevice, c->st),
+             "upload");
+        hipLaunchKernelGGL(k_lookup, dim3((unsigned)((lq.size() + 255) / 256)), dim3(256), 0, c->st, tp,
+                           (const i64*)b.p, (const int*)((char*)b.p + lq.size() * 8), (int)lq.size(),
+                           c->out128.as<u64>());
+        HIPT(hipGetLastError(), "launch lookup");
+    }
+    i64 nrec = 0;
+    if (!xps.empty()) {
+        // xparam.x holds the batch index of each X for the result slot
+        std::vector<XParam> upload = xps;
+        HIPT(c->xparam.ensure(upload.size() * sizeof(XParam)), "alloc params");
+        const auto th0 = std::chrono::steady_clock::now();
+        HIPT(hipMemcpyAsync(c->xparam.p, upload.data(), upload.size() * sizeof(XParam), hipMemcpyHostToDevice,
+                            c->st),
+             "upload params");
+        HIPT(hipStreamSynchronize(c->st), "upload params");
+        if (stats) stats->h2d_copy_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
+        HIPT(c->counts.ensure((size_t)slots * 8), "alloc counts");
+        HIPT(c->scan_tmp.ensure((size_t)std::max(scan_tmp_len(slots), scan_tmp_len(c->tnseg)) * 8), "alloc scan");
+        PartArgs pa{};
+        pa.xp = c->xparam.as<XParam>();
+        pa.nx = (int)xps.size();
+        pa.total_slots = slots;
+        pa.tp = tp;
+        pa.chunk = o.chunk > 0 ? o.chunk : tune_chunk();
+        pa.counts = c->counts.as<i64>();
+        pa.offsets = c->counts.as<i64>();
+        HIPT(hipEventRecord(c->ev[2], c->st), "event");
+        const unsigned gslots = (unsigned)((slots + 255) / 256);
+        hipLaunchKernelGGL(k_count, dim3(gslots), dim3(256), 0, c->st, pa);
+        HIPT(hipGetLastError(), "launch count");
+        HIPT(scan_exclusive_i64(c->counts.as<i64>(), c->counts.as<i64>(), slots, c->scan_tmp.as<i64>(),
+                                scan_tmp_len(slots), &nrec, c->st),
+             "scan counts");
+        HIPT(c->recs.ensure((size_t)nrec * sizeof(ChunkRec)), "alloc chunk records");
+        pa.recs = c->recs.as<ChunkRec>();
+        hipLaunchKernelGGL(k_expand, dim3(gslots), dim3(256), 0, c->st, pa);
+        HIPT(hipGetLastError(), "launch expand");
+        HIPT(hipEventRecord(c->ev[3], c->st), "event");
+        HIPT(c->ctr.ensure(64), "alloc counter");
+        HIPT(hipMemsetAsync(c->ctr.p, 0, 8, c->st), "clear counter");
+        if (o.record_cycles) HIPT(c->stamps.ensure((size_t)nrec * 4), "alloc stamps");
+        // records index X by its position in xps; the per-X sums land in
+        // comp_out and are remapped to batch positions below
+        QuotArgs qa{};
+        qa.xp = c->xparam.as<XParam>();
+        qa.tp = tp;
+        qa.recs = c->recs.as<ChunkRec>();
+        qa.nrec = nrec;
+        qa.next = c->ctr.as<unsigned long long>();
+        qa.stamps = o.record_cycles ? c->stamps.as<uint32_t>() : nullptr;
+        qa.dynamic = o.sched == MDR_SCHED_DYNAMIC ? 1 : 0;
+        static_assert(sizeof(ChunkRec) == 32, "record layout");
+        HIPT(c->comp_out.ensure(xps.size() * 16 + 16), "alloc accumulators");
+        HIPT(hipMemsetAsync(c->comp_out.p, 0, xps.size() * 16, c->st), "clear accumulators");
+        qa.out128 = c->comp_out.as<u64>();
+        const unsigned grid = (unsigned)(c->ncu * c->quot_blocks_per_cu);
+        hipLaunchKernelGGL(k_quot, dim3(grid), dim3(256), 0, c->st, qa);
+        HIPT(hipGetLastError(), "launch quotient kernel");
+        HIPT(hipEventRecord(c->ev[4], c->st), "event");
+        HIPT(hipEventSynchronize(c->ev[4]), "quotient phase");
+        std::vector<u64> acc_h(xps.size() * 2);
+        HIPT(hipMemcpy(acc_h.data(), c->comp_out.p, xps.size() * 16, hipMemcpyDeviceToHost), "download");
+        std::vector<u64> res(nx * 2);
+        HIPT(hipMemcpy(res.data(), c->out128.p, nx * 16, hipMemcpyDeviceToHost), "download");
+        for (size_t i = 0; i < xps.size(); ++i) {
+            res[2 * xidx[i]] = acc_h[2 * i];
+            res[2 * xidx[i] + 1] = acc_h[2 * i + 1];
+        }
+        for (size_t i = 0; i < nx; ++i) {
+            if (plans[i].kind == 0) { out[i] = plans[i].value; continue; }
+            const i64 lo = (i64)res[2 * i], hi = (i64)res[2 * i + 1];
+            if (hi != (lo >> 63)) return fail(c, MDR_ERR_RANGE, "result does not fit in 64 bits");
+            out[i] = lo;
+        }
+        if (stats) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+            stats->partition_ms += ms;
+            (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]);
+            stats->quotient_ms += ms;
+            stats->chunks += nrec;
+            stats->slots += slots;
+        }
+        if (o.record_cycles) {
+            c->last_stamps.resize((size_t)nrec);
+            c->last_recs.resize((size_t)nrec);
+            HIPT(hipMemcpy(c->last_stamps.data(), c->stamps.p, (size_t)nrec * 4, hipMemcpyDeviceToHost), "download");
+            HIPT(hipMemcpy(c->last_recs.data(), c->recs.p, (size_t)nrec * sizeof(ChunkRec), hipMemcpyDeviceToHost),
+                 "download");
+            c->last_xp = xps;
+        }
+    } else {
+        HIPT(hipStreamSynchronize(c->st), "lookups");
+        std::vector<u64> res(nx * 2);
+        HIPT(hipMemcpy(res.data(), c->out128.p, nx * 16, hipMemcpyDeviceToHost), "download");
+        for (size_t i = 0; i < nx; ++i) out[i] = plans[i].kind == 0 ? plans[i].value : (i64)res[2 * i];
+    }
+    c->last_nrec = nrec;
+    if (stats) {
+        stats->U = c->tU;
+        stats->u_dense = c->tud;
+        stats->mu_limit = c->tmu;
+        stats->W = c->tW;
+        if (!plans.empty()) {
+            const XPlan& p = plans.back();
+            stats->u = p.u;
+            stats->N = p.N;
+            stats->cx = p.cx;
+        }
+        stats->kernel_time = (stats->table_ms + stats->partition_ms + stats->quotient_ms) / 1000.0;
+        (void)t_call;
+    }
+    return MDR_OK;
+}
+
+}  // namespace
+
+// ---- C ABI --------------------------------------------------------------------------
+extern "C" {
+
+void mdr_default_opts(mdr_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->strategy = MDR_STRATEGY_AUTO;
+    o->partition = MDR_PARTITION_STATIC;
+    o->params = MDR_PARAMS_REFERENCE;
+    o->sched = MDR_SCHED_DYNAMIC;
+}
+
+mdr_status mdr_init(mdr_ctx** out, int device) {
+    if (!out) return MDR_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MDR_ERR_NODEVICE;
+    if (device < 0 || device >= n) return MDR_ERR_NODEVICE;
+    mdr_ctx* c = new mdr_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
+    int occ = 0;
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_quot, 256, 0);
+    if (e != hipSuccess) {
+        mdr_shutdown(c);
+        return MDR_ERR_HIP;
+    }
+    c->quot_blocks_per_cu = std::max(1, occ);
+    *out = c;
+    return MDR_OK;
+}
+
+void mdr_shutdown(mdr_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    DevBuf* bufs[] = {&c->Md, &c->cpl, &c->cck, &c->mu6, &c->segsum, &c->scan_tmp, &c->stencil, &c->wprimes,
+                      &c->wrcp, &c->wweights, &c->sqprimes, &c->xparam, &c->counts, &c->recs, &c->out128,
+                      &c->ctr, &c->stamps, &c->comp_out};
+    for (DevBuf* b : bufs) b->release();
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char* mdr_last_error(mdr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+mdr_status mdr_plan(mdr_ctx* c, long long X, const mdr_opts* opts, int* streaming, long long* u,
+                    long long* u_dense, long long* mu_limit, long long* N, double* cx) {
+    // Pure host logic: ctx may be NULL (then errors carry no message).
+    mdr_opts o;
+    mdr_default_opts(&o);
+    if (opts) o = *opts;
+    mdr_ctx scratch;
+    mdr_ctx* cc = c ? c : &scratch;
+    std::unique_lock<std::mutex> lk(cc->mu);
+    mdr_status s = validate_opts(cc, o);
+    if (s != MDR_OK) return s;
+    const XPlan p = plan_x(X, o);
+    if (streaming) *streaming = p.streaming;
+    if (u) *u = p.u;
+    if (u_dense) *u_dense = p.ud;
+    if (mu_limit) *mu_limit = p.mu_lim;
+    if (N) *N = p.N;
+    if (cx) *cx = p.cx;
+    return MDR_OK;
+}
+
+mdr_status mdr_mertens_batch(mdr_ctx* c, const long long* X, size_t n, const mdr_opts* opts, long long* out,
+                             mdr_stats* stats) {
+    if (!c || (!X && n) || (!out && n)) return MDR_ERR_INVALID;
+    mdr_opts o;
+    mdr_default_opts(&o);
+    if (opts) o = *opts;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    mdr_status s = validate_opts(c, o);
+    if (s != MDR_OK) return s;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const auto t0 = std::chrono::steady_clock::now();
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    std::vector<XPlan> plans(n);
+    for (size_t i = 0; i < n; ++i) plans[i] = plan_x(X[i], o);
+    s = run_batch(c, plans, o, out, stats);
+    if (stats) stats->total_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return s;
+}
+
+mdr_status mdr_mertens(mdr_ctx* c, long long X, const mdr_opts* opts, long long* out, mdr_stats* stats) {
+    return mdr_mertens_batch(c, &X, 1, opts, out, stats);
+}
+
+mdr_status mdr_table_build(mdr_ctx* c, long long U, long long u_dense, long long mu_limit) {
+    if (!c || U < 1 || u_dense < 0 || mu_limit < 0) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    return build_table(c, U, u_dense, mu_limit, nullptr);
+}
+
+}  // extern "C"
+
+// component kernels ------------------------------------------------------------------
+namespace {
+
+__global__ void k_read_M(TableParam tp, i64 first, i64 count, int32_t* out) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = get_M(tp, first + i);
+}
+__global__ void k_read_mu(TableParam tp, i64 first, i64 count, int8_t* out) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const i64 n = first + i;
+    int8_t m = 0;
+    if (n >= 1 && n <= tp.mu_lim) {
+        // mu(n) = mu(odd part) * (-1) when 2 || n, 0 when 4 | n; likewise for 3.
+        i64 v = n;
+        int sgn = 1;
+        bool zero = false;
+        if (v % 4 == 0) zero = true;
+        else if (v % 2 == 0) { v /= 2; sgn = -sgn; }
+        if (v % 9 == 0) zero = true;
+        else if (v % 3 == 0) { v /= 3; sgn = -sgn; }
+        if (!zero) m = (int8_t)(sgn * (mu6_code(tp.mu6, v / 3) - 1));
+    }
+    out[i] = m;
+}
+
+__global__ void k_component(TableParam tp, Seg sg, u64 y, int geval, int comb2, i64 u, double cx, u64* out128) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = ((i64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const i64 nw = ((i64)gridDim.x * blockDim.x) >> 6;
+    u64 lo = 0;
+    i64 hi = 0;
+    if (!geval) {
+        const i64 L = sg.count;
+        const i64 a = (i64)((u128)wave * (u64)L / (u64)nw), b = (i64)((u128)(wave + 1) * (u64)L / (u64)nw);
+        if (a < b) {
+            const i64 v = run_seg(tp, sg, y, (u64)a, (u64)b, lane);
+            add128(lo, hi, (u64)v, v >> 63);
+        }
+    } else {
+        XParam xp{};
+        xp.u = u;
+        Frame f{};
+        f.y = (i64)y;
+        f.coef = 1;
+        f.comb2 = comb2;
+        f.A = split_point((i64)y, cx, u);
+        f.ky = (i64)y / (f.A + 1);
+        if (comb2) {
+            f.B = split_point((i64)y >> 1, cx, u);
+            f.ky2 = ((i64)y >> 1) / (f.B + 1);
+        }
+        i64 L = 0;
+        const int ns = frame_nseg(f);
+        for (int i = 0; i < ns; ++i) L += frame_seg(f, xp, tp, i).count;
+        const i64 a = (i64)((u128)wave * (u64)L / (u64)nw), b = (i64)((u128)(wave + 1) * (u64)L / (u64)nw);
+        i64 s0 = 0;
+        for (int i = 0; i < ns; ++i) {
+            const Seg s = frame_seg(f, xp, tp, i);
+            const i64 s1 = s0 + s.count;
+            const i64 l = a > s0 ? a : s0, h = b < s1 ? b : s1;
+            if (l < h) {
+                const i64 v = s.sign * run_seg(tp, s, y, (u64)(l - s0), (u64)(h - s0), lane);
+                add128(lo, hi, (u64)v, v >> 63);
+            }
+            s0 = s1;
+        }
+        if (wave == 0 && lane == 0) {
+            const i128 cst = frame_const(f, tp);
+            add128(lo, hi, (u64)cst, (i64)(cst >> 64));
+        }
+    }
+    wave_reduce128(lo, hi);
+    if (lane == 0) atomic_add128(out128, lo, hi);
+}
+
+mdr_status run_component(mdr_ctx* c, const Seg& sg, i64 y, int geval, int comb2, i64 u, double cx, long long* lo,
+                         long long* hi) {
+    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    HIPT(c->comp_out.ensure(64), "alloc");
+    HIPT(hipMemsetAsync(c->comp_out.p, 0, 16, c->st), "clear");
+    hipLaunchKernelGGL(k_component, dim3(64), dim3(256), 0, c->st, c->tp(), sg, (u64)y, geval, comb2, u, cx,
+                       c->comp_out.as<u64>());
+    HIPT(hipGetLastError(), "launch component");
+    u64 r[2];
+    HIPT(hipMemcpyAsync(r, c->comp_out.p, 16, hipMemcpyDeviceToHost, c->st), "download");
+    HIPT(hipStreamSynchronize(c->st), "component");
+    *lo = (long long)r[0];
+    *hi = (long long)r[1];
+    return MDR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mdr_status mdr_table_read_M(mdr_ctx* c, long long first, long long count, int32_t* out) {
+    if (!c || count < 0 || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (first + count - 1 > c->tU) return fail(c, MDR_ERR_INVALID, "read beyond the table");
+    if (count == 0) return MDR_OK;
+    DevBuf tmp;
+    HIPT(tmp.ensure((size_t)count * 4), "alloc");
+    hipLaunchKernelGGL(k_read_M, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->st, c->tp(), first, count,
+                       tmp.as<int32_t>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp.p, (size_t)count * 4, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    tmp.release();
+    if (e != hipSuccess) return hip_fail(c, e, "read M");
+    return MDR_OK;
+}
+
+mdr_status mdr_table_read_mu(mdr_ctx* c, long long first, long long count, int8_t* out) {
+    if (!c || count < 0 || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (count == 0) return MDR_OK;
+    DevBuf tmp;
+    HIPT(tmp.ensure((size_t)count), "alloc");
+    hipLaunchKernelGGL(k_read_mu, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->st, c->tp(), first, count,
+                       tmp.as<int8_t>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp.p, (size_t)count, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    tmp.release();
+    if (e != hipSuccess) return hip_fail(c, e, "read mu");
+    return MDR_OK;
+}
+
+mdr_status mdr_s2_interval(mdr_ctx* c, int piece, long long first, long long last, long long y, long long* lo,
+                           long long* hi) {
+    if (!c || !lo || !hi || piece < 1 || piece > 12 || y < 0) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (first < 1) first = 1;
+    if (last > c->tmu) return fail(c, MDR_ERR_INVALID, "interval beyond the mu table");
+    Seg sg{};
+    sg.type = SEG_S2;
+    sg.piece = piece;
+    sg.sign = 1;
+    if (first <= last) {
+        sg.first = count_coprime6(first - 1);
+        sg.count = count_coprime6(last) - sg.first;
+    }
+    return run_component(c, sg, y, 0, 0, 0, 0, lo, hi);
+}
+
+mdr_status mdr_s1_sum(mdr_ctx* c, long long y, long long first, long long last, int stride, long long* lo,
+                      long long* hi) {
+    if (!c || !lo || !hi || (stride != 1 && stride != 2) || first < 1 || y < 0) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (first <= last && y / first > c->tU) return fail(c, MDR_ERR_INVALID, "quotient beyond the table");
+    // split at the tier boundary like the engine does
+    const i64 split = y / (c->tud + 1);
+    u64 rlo = 0;
+    i64 rhi = 0;
+    for (int comp = 1; comp >= 0; --comp) {
+        Seg sg{};
+        sg.type = comp ? SEG_S1_COMP : SEG_S1_DENSE;
+        sg.sign = 1;
+        sg.stride = stride;
+        i64 a = first, b = last;
+        if (comp) {
+            if (b > split) b = split;
+        } else if (a <= split) {
+            a = split + 1;
+            if (stride == 2 && ((a - first) & 1)) ++a;
+        }
+        if (a > b) continue;
+        if (comp && c->tW == 0) return fail(c, MDR_ERR_INVALID, "quotient beyond the dense table");
+        sg.first = a;
+        sg.count = (b - a) / stride + 1;
+        long long l, h;
+        mdr_status s = run_component(c, sg, y, 0, 0, 0, 0, &l, &h);
+        if (s != MDR_OK) return s;
+        add128(rlo, rhi, (u64)l, (i64)h);
+    }
+    *lo = (long long)rlo;
+    *hi = (long long)rhi;
+    return MDR_OK;
+}
+
+mdr_status mdr_g_eval(mdr_ctx* c, long long y, int comb2, long long u, double cx, long long* lo, long long* hi) {
+    if (!c || !lo || !hi || y < 4 || u < 2 || !(cx > 0)) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (u > c->tU) return fail(c, MDR_ERR_INVALID, "u beyond the table");
+    if ((i64)(cx * std::sqrt((double)y)) > c->tmu) return fail(c, MDR_ERR_INVALID, "split point beyond the mu table");
+    return run_component(c, Seg{}, y, 1, comb2 ? 1 : 0, u, cx, lo, hi);
+}
+
+}  // extern "C"

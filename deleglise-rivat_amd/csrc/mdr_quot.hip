@@ -1,0 +1,345 @@
+// Quotient phase: partition of every G evaluation into chunks, and the
+// persistent kernel that evaluates the S1 table sums and the S2 piecewise
+// mod-6 sums of each chunk.
+//
+// Reference: the term list and outer reduction (mertens_dr.hpp:1050-1071,
+// :1361-1380, :1469-1485, :1616-1632), eval_single_S / eval_comb2
+// (:1081-1354, :1490-1614), sum_dense_quotients (:88-128),
+// reconstruct_compressed (:741-752) and run_s2_fast (:484-617).  The
+// reference gives one worker per outer term; here every S1 run and S2 piece
+// of every term is cut into bounded chunks fed to persistent waves.
+#include <hip/hip_runtime.h>
+#include "mdr_device.hpp"
+#include "mdr_kernels.hpp"
+
+namespace mdr {
+
+namespace {
+
+__device__ __forceinline__ int find_x(const XParam* xp, int nx, i64 G) {
+    int lo = 0, hi = nx - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (xp[mid].slot_base <= G) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace
+
+// Total positions of one G evaluation (its segments concatenated in order).
+__device__ __forceinline__ i64 frame_positions(const Frame& f, const XParam& xp, const TableParam& tp) {
+    i64 L = 0;
+    const int ns = frame_nseg(f);
+    for (int i = 0; i < ns; ++i) L += frame_seg(f, xp, tp, i).count;
+    return L;
+}
+
+__device__ __forceinline__ i64 frame_chunks(const PartArgs& a, i64 L) {
+    const i64 c = (L + a.chunk - 1) / a.chunk;
+    return c > 0 ? c : 1;   // the constant still needs a record
+}
+
+__global__ void k_count(PartArgs a) {
+    const i64 G = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (G >= a.total_slots) return;
+    const int x = find_x(a.xp, a.nx, G);
+    const XParam xp = a.xp[x];
+    const Frame f = make_frame(xp, a.tp.mu6, G - xp.slot_base);
+    a.counts[G] = frame_nseg(f) > 0 ? frame_chunks(a, frame_positions(f, xp, a.tp)) : 0;
+}
+
+// Chunk j of c covers positions [j*L/c, (j+1)*L/c) of the concatenated
+// segments; the first chunk also adds the G evaluation's constant.
+__global__ void k_expand(PartArgs a) {
+    const i64 G = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (G >= a.total_slots) return;
+    const int x = find_x(a.xp, a.nx, G);
+    const XParam xp = a.xp[x];
+    const i64 g = G - xp.slot_base;
+    const Frame f = make_frame(xp, a.tp.mu6, g);
+    if (frame_nseg(f) == 0) return;
+    const i64 L = frame_positions(f, xp, a.tp);
+    const i64 c = frame_chunks(a, L);
+    const i64 o = a.offsets[G];
+    for (i64 j = 0; j < c; ++j) {
+        ChunkRec r;
+        r.x = (uint32_t)x;
+        r.g = (uint32_t)g;
+        r.seg = j == 0 ? 0x100u : 0u;
+        r.pad = 0;
+        r.p_lo = (u64)((u128)j * (u64)L / (u64)c);
+        r.p_hi = (u64)((u128)(j + 1) * (u64)L / (u64)c);
+        a.recs[o + j] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Chunk evaluators.  Lanes take consecutive positions (stride 64), so table
+// gathers of neighbouring n coalesce wherever y/n^2 is small.
+// ---------------------------------------------------------------------------
+template <int ST>
+__device__ i64 s1_dense(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) {
+    const double dy = (double)y;
+    const u64 nmax = nlo + (u64)ST * (cnt - 1);
+    if (nmax < 1431655765ull) {   // 3n <= 2^32: 32-bit remainder test
+        const uint32_t ylo = (uint32_t)y;
+        const uint32_t nb = (uint32_t)nlo;
+        int32_t s = 0;
+        for (uint32_t i = lane; i < (uint32_t)cnt; i += 64) {
+            const uint32_t n = nb + (uint32_t)ST * i;
+            const uint32_t q = qdiv32(ylo, dy, n, rcp_refined((double)n));
+            s += tp.Md[q];
+        }
+        return s;
+    }
+    i64 s = 0;
+    for (u64 i = lane; i < cnt; i += 64) {
+        const u64 n = nlo + (u64)ST * i;
+        s += tp.Md[qdiv_mid(y, dy, n, rcp_refined((double)n))];
+    }
+    return s;
+}
+
+template <int ST>
+__device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) {
+    const double dy = (double)y;
+    i64 s = 0;
+    for (u64 i = lane; i < cnt; i += 64) {
+        const u64 n = nlo + (u64)ST * i;
+        const u64 q = qdiv_mid(y, dy, n, rcp_refined((double)n));
+        const u64 off = q - (u64)tp.ud - 1;
+        const u64 w = off >> 6;
+        const u64 nz = tp.cpl[2 * w], ng = tp.cpl[2 * w + 1];
+        const u64 mask = ~0ull >> (63 - (off & 63));
+        s += tp.cck[w] + __popcll(nz & mask) - 2 * __popcll(ng & mask);
+    }
+    return s;
+}
+
+template <int P>
+__device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane) {
+    const double dy = (double)y;
+    // j >= (y >> 32) + 1 gives q < 2^32; the 32-bit remainder test also
+    // needs 3j <= 2^32.
+    const i64 j32 = (i64)(y >> 32) + 1;
+    u64 t_split = (u64)count_coprime6(j32 - 1);
+    if (j_of_t((i64)t_hi - 1) >= 1431655765ll) t_split = t_hi;
+    const u64 ts = t_split < t_lo ? t_lo : (t_split > t_hi ? t_hi : t_split);
+    i64 s = 0;
+    for (u64 t = t_lo + lane; t < ts; t += 64) {
+        const u64 j = (u64)j_of_t((i64)t);
+        const int code = mu6_code(tp.mu6, (i64)t);
+        const u64 q = qdiv_big(y, dy, j, rcp_refined((double)j));
+        const i64 v = piece_eval<P, u64>(q);
+        s += code == 2 ? v : (code == 0 ? -v : 0);
+    }
+    const uint32_t ylo = (uint32_t)y;
+    const uint32_t cnt = (uint32_t)(t_hi - ts);
+    const uint32_t tb = (uint32_t)ts;
+    for (uint32_t i = lane; i < cnt; i += 64) {
+        const uint32_t t = tb + i;
+        const uint32_t j = 3u * t + 1u + (t & 1u);
+        const int code = (int)((tp.mu6[t >> 4] >> (2 * (t & 15))) & 3u);
+        const uint32_t q = qdiv32(ylo, dy, j, rcp_refined((double)j));
+        const i64 v = piece_eval<P, uint32_t>(q);
+        s += code == 2 ? v : (code == 0 ? -v : 0);
+    }
+    return s;
+}
+
+__device__ i64 run_seg(const TableParam& tp, const Seg& sg, u64 y, u64 p_lo, u64 p_hi, int lane) {
+    const u64 cnt = p_hi - p_lo;
+    if (cnt == 0) return 0;
+    if (sg.type == SEG_S1_DENSE) {
+        const u64 nlo = (u64)sg.first + (u64)sg.stride * p_lo;
+        return sg.stride == 1 ? s1_dense<1>(tp, y, nlo, cnt, lane) : s1_dense<2>(tp, y, nlo, cnt, lane);
+    }
+    if (sg.type == SEG_S1_COMP) {
+        const u64 nlo = (u64)sg.first + (u64)sg.stride * p_lo;
+        return sg.stride == 1 ? s1_comp<1>(tp, y, nlo, cnt, lane) : s1_comp<2>(tp, y, nlo, cnt, lane);
+    }
+    const u64 t_lo = (u64)sg.first + p_lo, t_hi = (u64)sg.first + p_hi;
+    switch (sg.piece) {
+        case 1: return s2_run<1>(tp, y, t_lo, t_hi, lane);
+        case 2: return s2_run<2>(tp, y, t_lo, t_hi, lane);
+        case 3: return s2_run<3>(tp, y, t_lo, t_hi, lane);
+        case 4: return s2_run<4>(tp, y, t_lo, t_hi, lane);
+        case 5: return s2_run<5>(tp, y, t_lo, t_hi, lane);
+        case 6: return s2_run<6>(tp, y, t_lo, t_hi, lane);
+        case 7: return s2_run<7>(tp, y, t_lo, t_hi, lane);
+        case 8: return s2_run<8>(tp, y, t_lo, t_hi, lane);
+        case 9: return s2_run<9>(tp, y, t_lo, t_hi, lane);
+        case 10: return s2_run<10>(tp, y, t_lo, t_hi, lane);
+        case 11: return s2_run<11>(tp, y, t_lo, t_hi, lane);
+        default: return s2_run<12>(tp, y, t_lo, t_hi, lane);
+    }
+}
+
+namespace {
+__device__ __forceinline__ i64 uniform64(i64 v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(u64)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((u64)v >> 32));
+    return (i64)(((u64)hi << 32) | lo);
+}
+__device__ __forceinline__ i64 fetch_next(unsigned long long* ctr, int lane) {
+    unsigned long long v = 0;
+    if (lane == 0) v = atomicAdd(ctr, 1ull);
+    return uniform64((i64)__shfl(v, 0, 64));
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = ((i64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = ((i64)gridDim.x * blockDim.x) >> 6;
+    u64 acc_lo = 0;
+    i64 acc_hi = 0;
+    i64 cur_x = -1;
+    i64 r = a.dynamic ? fetch_next(a.next, lane) : uniform64(wave);
+    while (r < a.nrec) {
+        const ChunkRec rec = a.recs[r];
+        if ((i64)rec.x != cur_x) {
+            if (cur_x >= 0) {
+                wave_reduce128(acc_lo, acc_hi);
+                if (lane == 0) atomic_add128(a.out128 + 2 * cur_x, acc_lo, acc_hi);
+                acc_lo = 0;
+                acc_hi = 0;
+            }
+            cur_x = rec.x;
+        }
+        const long long t0 = a.stamps ? clock64() : 0;
+        const XParam xp = a.xp[rec.x];
+        const Frame f = make_frame(xp, a.tp.mu6, rec.g);
+        const int ns = frame_nseg(f);
+        i64 s0 = 0;
+        for (int i = 0; i < ns && s0 < (i64)rec.p_hi; ++i) {
+            const Seg sg = frame_seg(f, xp, a.tp, i);
+            const i64 s1 = s0 + sg.count;
+            const i64 lo = (i64)rec.p_lo > s0 ? (i64)rec.p_lo : s0;
+            const i64 hi = (i64)rec.p_hi < s1 ? (i64)rec.p_hi : s1;
+            if (lo < hi) {
+                const i64 part = run_seg(a.tp, sg, (u64)f.y, (u64)(lo - s0), (u64)(hi - s0), lane);
+                const i64 v = (f.coef * sg.sign) * part;
+                add128(acc_lo, acc_hi, (u64)v, v >> 63);
+            }
+            s0 = s1;
+        }
+        if ((rec.seg & 0x100u) && lane == 0) {
+            const i128 c = (i128)f.coef * frame_const(f, a.tp);
+            add128(acc_lo, acc_hi, (u64)c, (i64)(c >> 64));
+        }
+        if (a.stamps && lane == 0) {
+            const long long dt = clock64() - t0;
+            a.stamps[r] = (uint32_t)(dt > 0xffffffffll ? 0xffffffffll : dt);
+        }
+        r = a.dynamic ? fetch_next(a.next, lane) : r + nwaves;
+    }
+    if (cur_x >= 0) {
+        wave_reduce128(acc_lo, acc_hi);
+        if (lane == 0) atomic_add128(a.out128 + 2 * cur_x, acc_lo, acc_hi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of i64 (three passes: tile sums, scan of tile sums, tiles).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int SCAN_T = 256;
+constexpr int SCAN_PER = 8;
+constexpr int SCAN_TILE = SCAN_T * SCAN_PER;
+
+__device__ i64 block_excl_scan(i64 v, i64* total) {
+    __shared__ i64 wsum[SCAN_T / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    i64 inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const i64 o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    i64 off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_T / 64; ++w) {
+        if (w < wave) off += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+}  // namespace
+
+__global__ void k_scan_reduce(const i64* in, i64 n, i64* tsum) {
+    const i64 base = (i64)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    i64 s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; ++i) s += (base + i < n) ? in[base + i] : 0;
+    i64 tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_top(i64* tsum, i64 nt, i64* total) {
+    i64 carry = 0;
+    for (i64 b = 0; b < nt; b += SCAN_TILE) {
+        const i64 base = b + threadIdx.x * SCAN_PER;
+        i64 v[SCAN_PER];
+        i64 s = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_PER; ++i) { v[i] = (base + i < nt) ? tsum[base + i] : 0; s += v[i]; }
+        i64 tot;
+        i64 ex = block_excl_scan(s, &tot) + carry;
+#pragma unroll
+        for (int i = 0; i < SCAN_PER; ++i) {
+            if (base + i < nt) tsum[base + i] = ex;
+            ex += v[i];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scan_down(const i64* in, i64* out, i64 n, const i64* tsum) {
+    const i64 base = (i64)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    i64 v[SCAN_PER];
+    i64 s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; ++i) { v[i] = (base + i < n) ? in[base + i] : 0; s += v[i]; }
+    i64 tot;
+    i64 ex = block_excl_scan(s, &tot) + tsum[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; ++i) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+}
+
+i64 scan_tmp_len(i64 n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+// d_tmp must hold scan_tmp_len(n) values; d_tmp[last] receives the total.
+hipError_t scan_exclusive_i64(const i64* in, i64* out, i64 n, i64* d_tmp, i64 tmp_len, i64* total,
+                              hipStream_t st) {
+    if (n <= 0) {
+        if (total) *total = 0;
+        return hipSuccess;
+    }
+    const i64 nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tmp_len < nt + 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(SCAN_T), 0, st, in, n, d_tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, d_tmp, nt, d_tmp + nt);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nt), dim3(SCAN_T), 0, st, in, out, n, d_tmp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (total) {
+        e = hipMemcpyAsync(total, d_tmp + nt, sizeof(i64), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(st);
+    }
+    return e;
+}
+
+}  // namespace mdr

@@ -1,0 +1,21 @@
+// Tuned GPU parameters: the cx/fx split and the chunk length.  Values come
+// from the autotuner's cache when present, else from the built-in defaults.
+#ifndef MDR_TUNE_HPP
+#define MDR_TUNE_HPP
+
+#include "mdr_common.hpp"
+
+namespace mdr {
+
+struct TuneChoice {
+    i64 u;      // sieve cutoff
+    i64 ud;     // dense tier limit (multiple of 64)
+    double cx;  // S1/S2 split factor
+};
+
+TuneChoice tune_lookup(i64 X, bool streaming);
+i64 tune_chunk();
+
+}  // namespace mdr
+
+#endif

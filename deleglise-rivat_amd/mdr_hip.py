@@ -1,0 +1,246 @@
+"""ctypes binding of libmdr_hip.so (the C ABI declared in include/mertens_hip.h).
+
+The binding is a thin host-side mirror of the reference entry points
+(mertens_dr::DelégliseRivatEngine::compute_mertens / compute_compressed,
+mertens_dr.hpp:968 and :1442) for tests and the benchmark.  It never computes
+anything itself: when the HIP library is missing it raises, it does not fall
+back to a CPU path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdr_hip.so")
+
+STRATEGY_AUTO, STRATEGY_DENSE, STRATEGY_STREAMING, STRATEGY_FORCE_STREAMING = 0, 1, 2, 3
+PARTITION_STATIC, PARTITION_LEARNED = 0, 1
+PARAMS_REFERENCE, PARAMS_TUNED = 0, 1
+SCHED_DYNAMIC, SCHED_STATIC = 0, 1
+
+STATUS = {0: "MDR_OK", 1: "MDR_ERR_INVALID", 2: "MDR_ERR_HIP", 3: "MDR_ERR_NOMEM",
+          4: "MDR_ERR_NODEVICE", 5: "MDR_ERR_RANGE", 6: "MDR_ERR_NOTABLE"}
+
+# Every symbol include/mertens_hip.h declares.
+EXPORTS = ["mdr_default_opts", "mdr_init", "mdr_shutdown", "mdr_last_error", "mdr_mertens",
+           "mdr_mertens_batch", "mdr_plan", "mdr_table_build", "mdr_table_read_M",
+           "mdr_table_read_mu", "mdr_s2_interval", "mdr_s1_sum", "mdr_g_eval"]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("strategy", ctypes.c_int), ("cx", ctypes.c_double), ("u_cap", ctypes.c_longlong),
+                ("fx", ctypes.c_double), ("u_cap_dense", ctypes.c_longlong), ("fx_set", ctypes.c_int),
+                ("u_cap_dense_set", ctypes.c_int), ("partition", ctypes.c_int), ("params", ctypes.c_int),
+                ("sched", ctypes.c_int), ("host_threads", ctypes.c_int), ("chunk", ctypes.c_longlong),
+                ("record_cycles", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("host_sieve_time", ctypes.c_double), ("h2d_copy_time", ctypes.c_double),
+                ("kernel_time", ctypes.c_double), ("d2h_copy_time", ctypes.c_double),
+                ("total_time", ctypes.c_double), ("table_ms", ctypes.c_double),
+                ("partition_ms", ctypes.c_double), ("quotient_ms", ctypes.c_double),
+                ("table_reused", ctypes.c_int), ("U", ctypes.c_longlong), ("u_dense", ctypes.c_longlong),
+                ("mu_limit", ctypes.c_longlong), ("W", ctypes.c_longlong), ("u", ctypes.c_longlong),
+                ("N", ctypes.c_longlong), ("slots", ctypes.c_longlong), ("chunks", ctypes.c_longlong),
+                ("cx", ctypes.c_double), ("s1_terms", ctypes.c_longlong), ("s2_terms", ctypes.c_longlong),
+                ("table_bytes", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class MdrError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libmdr_hip.so; raise if it is absent (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MdrError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    ll, i32, dbl = ctypes.c_longlong, ctypes.c_int, ctypes.c_double
+    vp = ctypes.c_void_p
+    lib.mdr_default_opts.argtypes = [P(Opts)]
+    lib.mdr_default_opts.restype = None
+    lib.mdr_init.argtypes = [P(vp), i32]
+    lib.mdr_init.restype = i32
+    lib.mdr_shutdown.argtypes = [vp]
+    lib.mdr_shutdown.restype = None
+    lib.mdr_last_error.argtypes = [vp]
+    lib.mdr_last_error.restype = ctypes.c_char_p
+    lib.mdr_mertens.argtypes = [vp, ll, P(Opts), P(ll), P(Stats)]
+    lib.mdr_mertens.restype = i32
+    lib.mdr_mertens_batch.argtypes = [vp, P(ll), ctypes.c_size_t, P(Opts), P(ll), P(Stats)]
+    lib.mdr_mertens_batch.restype = i32
+    lib.mdr_plan.argtypes = [vp, ll, P(Opts), P(i32), P(ll), P(ll), P(ll), P(ll), P(dbl)]
+    lib.mdr_plan.restype = i32
+    lib.mdr_table_build.argtypes = [vp, ll, ll, ll]
+    lib.mdr_table_build.restype = i32
+    lib.mdr_table_read_M.argtypes = [vp, ll, ll, P(ctypes.c_int32)]
+    lib.mdr_table_read_M.restype = i32
+    lib.mdr_table_read_mu.argtypes = [vp, ll, ll, P(ctypes.c_int8)]
+    lib.mdr_table_read_mu.restype = i32
+    lib.mdr_s2_interval.argtypes = [vp, i32, ll, ll, ll, P(ll), P(ll)]
+    lib.mdr_s2_interval.restype = i32
+    lib.mdr_s1_sum.argtypes = [vp, ll, ll, ll, i32, P(ll), P(ll)]
+    lib.mdr_s1_sum.restype = i32
+    lib.mdr_g_eval.argtypes = [vp, ll, i32, ll, dbl, P(ll), P(ll)]
+    lib.mdr_g_eval.restype = i32
+    _lib = lib
+    return lib
+
+
+def _i128(lo, hi):
+    return (hi.value << 64) | (lo.value & ((1 << 64) - 1))
+
+
+def make_opts(strategy=STRATEGY_AUTO, cx=0.0, u_cap=0, fx=0.0, u_cap_dense=0, fx_set=0,
+              u_cap_dense_set=0, partition=PARTITION_STATIC, params=PARAMS_REFERENCE,
+              sched=SCHED_DYNAMIC, chunk=0, record_cycles=0):
+    o = Opts()
+    load().mdr_default_opts(ctypes.byref(o))
+    o.strategy, o.cx, o.u_cap, o.fx, o.u_cap_dense = strategy, cx, u_cap, fx, u_cap_dense
+    o.fx_set, o.u_cap_dense_set, o.partition, o.params = fx_set, u_cap_dense_set, partition, params
+    o.sched, o.chunk, o.record_cycles = sched, chunk, record_cycles
+    return o
+
+
+def opts_from_env(env=None):
+    """Parse the reference's strategy controls exactly as mertens_dr.hpp does
+    (MERTENS_USE_CHMT :1027/:1430-1435, MERTENS_CX :915-918,
+    MERTENS_CHMT_U_CAP :1412-1415, MERTENS_FX :944-947, MERTENS_U_CAP :957-960)."""
+    env = os.environ if env is None else env
+    kw = {}
+    if "MERTENS_USE_CHMT" in env:
+        v = env["MERTENS_USE_CHMT"]
+        kw["strategy"] = STRATEGY_STREAMING if v[:1] in ("1", "y", "Y", "t", "T") else STRATEGY_DENSE
+    if "MERTENS_CX" in env:
+        v = _atof(env["MERTENS_CX"])
+        if 0.1 < v < 5.0:
+            kw["cx"] = v
+    if "MERTENS_CHMT_U_CAP" in env:
+        v = _atoll(env["MERTENS_CHMT_U_CAP"])
+        if 100000000 <= v <= 10000000000:
+            kw["u_cap"] = v
+    if "MERTENS_FX" in env:
+        kw["fx_set"] = 1
+        v = _atof(env["MERTENS_FX"])
+        if 0.1 <= v <= 3.0:
+            kw["fx"] = v
+    if "MERTENS_U_CAP" in env:
+        kw["u_cap_dense_set"] = 1
+        v = _atoll(env["MERTENS_U_CAP"])
+        if 100000000 <= v <= 4000000000:
+            kw["u_cap_dense"] = v
+    return kw
+
+
+def _atof(s):
+    """C atof: longest valid leading float prefix, 0.0 when none."""
+    import re
+    m = re.match(r"\s*[+-]?(\d+\.?\d*([eE][+-]?\d+)?|\.\d+([eE][+-]?\d+)?)", s)
+    return float(m.group(0)) if m else 0.0
+
+
+def _atoll(s):
+    """C atoll: optional sign and leading digits, 0 when none."""
+    import re
+    m = re.match(r"\s*[+-]?\d+", s)
+    return int(m.group(0)) if m else 0
+
+
+class Engine:
+    """One GPU context (mdr_init / mdr_shutdown)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.ctx = ctypes.c_void_p()
+        st = self.lib.mdr_init(ctypes.byref(self.ctx), device)
+        if st != 0:
+            raise MdrError(f"mdr_init failed: {STATUS.get(st, st)}")
+
+    def close(self):
+        if self.ctx:
+            self.lib.mdr_shutdown(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st, what):
+        if st != 0:
+            msg = self.lib.mdr_last_error(self.ctx).decode(errors="replace")
+            raise MdrError(f"{what}: {STATUS.get(st, st)}: {msg}")
+
+    def mertens(self, X, opts=None, **kw):
+        o = opts if opts is not None else make_opts(**kw)
+        out = ctypes.c_longlong()
+        stats = Stats()
+        self._check(self.lib.mdr_mertens(self.ctx, X, ctypes.byref(o), ctypes.byref(out), ctypes.byref(stats)),
+                    f"mdr_mertens({X})")
+        self.last_stats = stats
+        return out.value
+
+    def batch(self, xs, opts=None, **kw):
+        o = opts if opts is not None else make_opts(**kw)
+        n = len(xs)
+        arr = (ctypes.c_longlong * n)(*xs)
+        out = (ctypes.c_longlong * n)()
+        stats = Stats()
+        self._check(self.lib.mdr_mertens_batch(self.ctx, arr, n, ctypes.byref(o), out, ctypes.byref(stats)),
+                    "mdr_mertens_batch")
+        self.last_stats = stats
+        return list(out)
+
+    def plan(self, X, opts=None, **kw):
+        o = opts if opts is not None else make_opts(**kw)
+        s, u, ud, mu, N, cx = (ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong(),
+                               ctypes.c_longlong(), ctypes.c_double())
+        self._check(self.lib.mdr_plan(self.ctx, X, ctypes.byref(o), *(ctypes.byref(v) for v in (s, u, ud, mu, N, cx))),
+                    "mdr_plan")
+        return dict(streaming=bool(s.value), u=u.value, u_dense=ud.value, mu_limit=mu.value, N=N.value, cx=cx.value)
+
+    def table_build(self, U, u_dense, mu_limit):
+        self._check(self.lib.mdr_table_build(self.ctx, U, u_dense, mu_limit), "mdr_table_build")
+
+    def read_M(self, first, count):
+        import numpy as np
+        out = np.empty(count, dtype=np.int32)
+        self._check(self.lib.mdr_table_read_M(self.ctx, first, count,
+                                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "read_M")
+        return out
+
+    def read_mu(self, first, count):
+        import numpy as np
+        out = np.empty(count, dtype=np.int8)
+        self._check(self.lib.mdr_table_read_mu(self.ctx, first, count,
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))), "read_mu")
+        return out
+
+    def s2_interval(self, piece, first, last, y):
+        lo, hi = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self.lib.mdr_s2_interval(self.ctx, piece, first, last, y, ctypes.byref(lo), ctypes.byref(hi)),
+                    "s2_interval")
+        return _i128(lo, hi)
+
+    def s1_sum(self, y, first, last, stride):
+        lo, hi = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self.lib.mdr_s1_sum(self.ctx, y, first, last, stride, ctypes.byref(lo), ctypes.byref(hi)),
+                    "s1_sum")
+        return _i128(lo, hi)
+
+    def g_eval(self, y, comb2, u, cx):
+        lo, hi = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self.lib.mdr_g_eval(self.ctx, y, 1 if comb2 else 0, u, cx, ctypes.byref(lo), ctypes.byref(hi)),
+                    "g_eval")
+        return _i128(lo, hi)

@@ -1,0 +1,131 @@
+/*
+ * C ABI of the MI355X Deleglise-Rivat Mertens engine (libmdr_hip.so).
+ *
+ * Plain C types only: 64-bit integers, doubles, pointers and sizes. Nothing
+ * throws across this boundary; every call returns an mdr_status and writes its
+ * results through caller-owned pointers.  A context owns one GPU's device
+ * buffers (tables are cached across calls) and serialises the calls made on
+ * it; use one context per host thread.
+ *
+ * Reference interfaces each entry point replaces:
+ *   mdr_mertens        mertens_dr::DelégliseRivatEngine::compute_mertens
+ *                      (mertens_dr.hpp:968) and compute_compressed (:1442);
+ *                      mertens_cuda::CudaDelégliseRivatEngine::compute_mertens
+ *                      with its TimingStats out-pointer (mertens_cuda.cuh:410-416, :441)
+ *   mdr_mertens_batch  the same, for many X sharing one table (no reference
+ *                      counterpart beyond repeated compute_mertens calls)
+ *   mdr_table_*        SieveTable / CompressedMertensTable get_M and get_mu
+ *                      (mertens_dr.hpp:404-425, :707-716)
+ *   mdr_s2_interval    run_s2_fast<Piece> (mertens_dr.hpp:484-617)
+ *   mdr_s1_sum         sum_dense_quotients<Stride> (mertens_dr.hpp:88-128) and the
+ *                      compressed-tier loops (:1499-1504)
+ *   mdr_g_eval         eval_single_S / eval_comb2 (mertens_dr.hpp:1081-1354)
+ *   mdr_plan           choose_cx / choose_sieve_limit / choose_compressed_limit and
+ *                      the strategy switch (mertens_dr.hpp:914-962, :1027-1041,
+ *                      :1401-1461)
+ */
+#ifndef MERTENS_HIP_H
+#define MERTENS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdr_ctx mdr_ctx;
+
+typedef enum {
+    MDR_OK = 0,
+    MDR_ERR_INVALID = 1,   /* bad argument or option out of its documented range */
+    MDR_ERR_HIP = 2,       /* a HIP runtime call failed (see mdr_last_error) */
+    MDR_ERR_NOMEM = 3,     /* device allocation failed */
+    MDR_ERR_NODEVICE = 4,  /* no usable GPU */
+    MDR_ERR_RANGE = 5,     /* result or work size outside the supported range */
+    MDR_ERR_NOTABLE = 6    /* component call made before a table was built */
+} mdr_status;
+
+/* strategy: MERTENS_USE_CHMT (mertens_dr.hpp:1027-1029, :1421-1435) */
+enum {
+    MDR_STRATEGY_AUTO = 0,       /* variable unset: streaming from 1e11 unless fx/u_cap_dense set */
+    MDR_STRATEGY_DENSE = 1,      /* variable set, not 1/y/Y/t/T */
+    MDR_STRATEGY_STREAMING = 2,  /* variable set to 1/y/Y/t/T: streaming from 1e11 */
+    MDR_STRATEGY_FORCE_STREAMING = 3  /* compute_compressed: streaming for X > 1e5 */
+};
+enum { MDR_PARTITION_STATIC = 0, MDR_PARTITION_LEARNED = 1 };
+enum { MDR_PARAMS_REFERENCE = 0, MDR_PARAMS_TUNED = 1 };
+enum { MDR_SCHED_DYNAMIC = 0, MDR_SCHED_STATIC = 1 };
+
+typedef struct {
+    int strategy;             /* MDR_STRATEGY_* */
+    double cx;                /* MERTENS_CX, 0 = default; accepted 0.1 < cx < 5.0 */
+    long long u_cap;          /* MERTENS_CHMT_U_CAP, 0 = default 8e9; accepted 1e8..1e10 */
+    double fx;                /* MERTENS_FX, 0 = default 0.32; accepted 0.1..3.0 */
+    long long u_cap_dense;    /* MERTENS_U_CAP, 0 = default 1.5e9; accepted 1e8..4e9 */
+    int fx_set;               /* MERTENS_FX present in the environment (disables auto streaming) */
+    int u_cap_dense_set;      /* MERTENS_U_CAP present in the environment (same) */
+    int partition;            /* MDR_PARTITION_* */
+    int params;               /* MDR_PARAMS_*: reference parameters or the autotuned GPU split */
+    int sched;                /* MDR_SCHED_* */
+    int host_threads;         /* accepted for interface parity; the GPU path ignores it */
+    long long chunk;          /* static chunk length in positions, 0 = autotuned default */
+    int record_cycles;        /* 1: record per-chunk cycle counts (training data) */
+} mdr_opts;
+
+typedef struct {
+    /* mirrors CudaDelégliseRivatEngine::TimingStats (seconds), all host-timed */
+    double host_sieve_time;   /* always 0: the sieve runs on the GPU */
+    double h2d_copy_time;     /* parameter upload */
+    double kernel_time;       /* table build + partition + quotient phase (GPU events) */
+    double d2h_copy_time;     /* result download */
+    double total_time;
+    /* GPU phase times in milliseconds (hipEvents on the engine's stream) */
+    double table_ms, partition_ms, quotient_ms;
+    int table_reused;
+    long long U, u_dense, mu_limit, W;   /* table actually used */
+    long long u, N, slots, chunks;       /* last X of the call */
+    double cx;
+    long long s1_terms, s2_terms;        /* quotient positions (sum over the call) */
+    double table_bytes;                  /* bytes of the table arrays written */
+} mdr_stats;
+
+void mdr_default_opts(mdr_opts* opts);
+mdr_status mdr_init(mdr_ctx** ctx, int device);
+void mdr_shutdown(mdr_ctx* ctx);
+const char* mdr_last_error(mdr_ctx* ctx);
+
+/* M(X) for one X; *out receives the exact signed 64-bit value. */
+mdr_status mdr_mertens(mdr_ctx* ctx, long long X, const mdr_opts* opts, long long* out,
+                       mdr_stats* stats);
+/* M(X[i]) for n values sharing one table sized to the largest cutoff. */
+mdr_status mdr_mertens_batch(mdr_ctx* ctx, const long long* X, size_t n, const mdr_opts* opts,
+                             long long* out, mdr_stats* stats);
+
+/* Parameters the engine would use for X (reference rules or tuned).  Host-only;
+ * ctx may be NULL. */
+mdr_status mdr_plan(mdr_ctx* ctx, long long X, const mdr_opts* opts, int* streaming, long long* u,
+                    long long* u_dense, long long* mu_limit, long long* N, double* cx);
+
+/* Component entry points (parity tests).  mdr_table_build sieves n = 1..U
+ * with the dense tier up to u_dense (a multiple of 64 when below U) and packed
+ * mu up to mu_limit; the reads copy M(first..first+count-1) and mu(...) to host. */
+mdr_status mdr_table_build(mdr_ctx* ctx, long long U, long long u_dense, long long mu_limit);
+mdr_status mdr_table_read_M(mdr_ctx* ctx, long long first, long long count, int32_t* out);
+mdr_status mdr_table_read_mu(mdr_ctx* ctx, long long first, long long count, int8_t* out);
+/* sum over j in [first, last], gcd(j,6)=1 of mu(j)*piece(floor(y/j)); piece 1..8 are
+ * Piece1..8, 9..12 SinglePiece1..4.  128-bit result as (lo, hi). */
+mdr_status mdr_s2_interval(mdr_ctx* ctx, int piece, long long first, long long last, long long y,
+                           long long* lo, long long* hi);
+/* sum over n = first, first+stride, ... <= last of M(floor(y/n)); stride 1 or 2. */
+mdr_status mdr_s1_sum(mdr_ctx* ctx, long long y, long long first, long long last, int stride,
+                      long long* lo, long long* hi);
+/* G(y) (comb2 = 0) or G(y) - G(floor(y/2)) (comb2 = 1) with cutoff u and split cx. */
+mdr_status mdr_g_eval(mdr_ctx* ctx, long long y, int comb2, long long u, double cx, long long* lo,
+                      long long* hi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
