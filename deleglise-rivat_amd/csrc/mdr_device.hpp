@@ -110,7 +110,7 @@ __device__ __forceinline__ int32_t get_M(const TableParam& tp, i64 q) {
     const i64 w = off >> 6;
     const u64 nz = tp.cpl[2 * w], ng = tp.cpl[2 * w + 1];
     const u64 mask = ~0ull >> (63 - (off & 63));
-    return tp.cck[w] + __popcll(nz & mask) - 2 * __popcll(ng & mask);
+    return tp.cck[w] + (int)__popcll(nz & mask) - 2 * (int)__popcll(ng & mask);
 }
 
 // ---------------------------------------------------------------------------

@@ -113,7 +113,7 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
         const u64 w = off >> 6;
         const u64 nz = tp.cpl[2 * w], ng = tp.cpl[2 * w + 1];
         const u64 mask = ~0ull >> (63 - (off & 63));
-        s += tp.cck[w] + __popcll(nz & mask) - 2 * __popcll(ng & mask);
+        s += tp.cck[w] + (int)__popcll(nz & mask) - 2 * (int)__popcll(ng & mask);
     }
     return s;
 }

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
         const int8_t v = (int8_t)acc[g * 64 + lane];
         const u64 nz = __ballot(v != 0);
         const u64 ng = __ballot(v < 0);
-        if (lane == 0) gsum[g] = __popcll(nz) - 2 * __popcll(ng);
+        if (lane == 0) gsum[g] = (int)__popcll(nz) - 2 * (int)__popcll(ng);
         const u64 n0 = first + (u64)g * 64;
         if (n0 > (u64)a.ud && n0 <= (u64)a.U && lane < 2) {
             const i64 w = (i64)(n0 - (u64)a.ud - 1) >> 6;
