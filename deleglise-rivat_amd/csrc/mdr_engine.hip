@@ -276,13 +276,13 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
 }
 
 // ---- table -----------------------------------------------------------------------
-mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats) {
+mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, bool reuse = true) {
     if (U < 1) U = 1;
     if (ud > U) ud = U;
     if (ud < U && (ud % 64) != 0) return fail(c, MDR_ERR_INVALID, "u_dense must be a multiple of 64 below U");
     if (mu_lim > U) return fail(c, MDR_ERR_INVALID, "mu_limit above U");
     if (mu_lim < 1) mu_lim = 1;
-    if (c->tab_valid && c->tU == U && c->tud == ud && c->tmu == mu_lim) {
+    if (reuse && c->tab_valid && c->tU == U && c->tud == ud && c->tmu == mu_lim) {
         if (stats) stats->table_reused = 1;
         return MDR_OK;
     }
@@ -387,7 +387,7 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
     bool need_table = false;
     for (const XPlan& p : plans) need_table |= p.kind != 0;
     if (need_table) {
-        mdr_status s = build_table(c, U, ud, mu_lim, stats);
+        mdr_status s = build_table(c, U, ud, mu_lim, stats, !o.no_table_reuse);
         if (s != MDR_OK) return s;
     }
     const TableParam tp = c->tp();
@@ -449,6 +449,9 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
         pa.chunk = o.chunk > 0 ? o.chunk : tune_chunk();
         pa.counts = c->counts.as<i64>();
         pa.offsets = c->counts.as<i64>();
+        HIPT(c->ctr.ensure(64), "alloc counter");
+        HIPT(hipMemsetAsync(c->ctr.p, 0, 64, c->st), "clear counters");
+        pa.type_pos = c->ctr.as<unsigned long long>() + 1;
         HIPT(hipEventRecord(c->ev[2], c->st), "event");
         const unsigned gslots = (unsigned)((slots + 255) / 256);
         hipLaunchKernelGGL(k_count, dim3(gslots), dim3(256), 0, c->st, pa);
@@ -460,9 +463,7 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
         pa.recs = c->recs.as<ChunkRec>();
         hipLaunchKernelGGL(k_expand, dim3(gslots), dim3(256), 0, c->st, pa);
         HIPT(hipGetLastError(), "launch expand");
-        HIPT(hipEventRecord(c->ev[3], c->st), "event");
-        HIPT(c->ctr.ensure(64), "alloc counter");
-        HIPT(hipMemsetAsync(c->ctr.p, 0, 8, c->st), "clear counter");
+
         if (o.record_cycles) HIPT(c->stamps.ensure((size_t)nrec * 4), "alloc stamps");
         // records index X by its position in xps; the per-X sums land in
         // comp_out and are remapped to batch positions below
@@ -479,6 +480,7 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
         HIPT(hipMemsetAsync(c->comp_out.p, 0, xps.size() * 16, c->st), "clear accumulators");
         qa.out128 = c->comp_out.as<u64>();
         const unsigned grid = (unsigned)(c->ncu * c->quot_blocks_per_cu);
+        HIPT(hipEventRecord(c->ev[3], c->st), "event");
         hipLaunchKernelGGL(k_quot, dim3(grid), dim3(256), 0, c->st, qa);
         HIPT(hipGetLastError(), "launch quotient kernel");
         HIPT(hipEventRecord(c->ev[4], c->st), "event");
@@ -505,6 +507,11 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
             stats->quotient_ms += ms;
             stats->chunks += nrec;
             stats->slots += slots;
+            unsigned long long tp3[3];
+            HIPT(hipMemcpy(tp3, c->ctr.as<unsigned long long>() + 1, sizeof tp3, hipMemcpyDeviceToHost), "download");
+            stats->s1_terms += (long long)(tp3[0] + tp3[1]);
+            stats->s1_comp_terms += (long long)tp3[1];
+            stats->s2_terms += (long long)tp3[2];
         }
         if (o.record_cycles) {
             c->last_stamps.resize((size_t)nrec);

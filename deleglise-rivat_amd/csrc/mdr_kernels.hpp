@@ -42,6 +42,7 @@ struct PartArgs {
     const float* cost_rate;    // learned: predicted cycles per position per segment class (nullptr = static)
     double cost_target;        // learned: target cycles per chunk
     i64* counts;               // per slot chunk count (k_count output)
+    unsigned long long* type_pos;  // [3]: S1 dense, S1 compressed, S2 positions
     const i64* offsets;        // per slot first record (k_expand input)
     ChunkRec* recs;
 };

@@ -43,11 +43,27 @@ __device__ __forceinline__ i64 frame_chunks(const PartArgs& a, i64 L) {
 
 __global__ void k_count(PartArgs a) {
     const i64 G = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (G >= a.total_slots) return;
-    const int x = find_x(a.xp, a.nx, G);
-    const XParam xp = a.xp[x];
-    const Frame f = make_frame(xp, a.tp.mu6, G - xp.slot_base);
-    a.counts[G] = frame_nseg(f) > 0 ? frame_chunks(a, frame_positions(f, xp, a.tp)) : 0;
+    u64 pos[3] = {0, 0, 0};
+    if (G < a.total_slots) {
+        const int x = find_x(a.xp, a.nx, G);
+        const XParam xp = a.xp[x];
+        const Frame f = make_frame(xp, a.tp.mu6, G - xp.slot_base);
+        const int ns = frame_nseg(f);
+        i64 L = 0;
+        for (int i = 0; i < ns; ++i) {
+            const Seg s = frame_seg(f, xp, a.tp, i);
+            L += s.count;
+            pos[s.type == SEG_S1_DENSE ? 0 : s.type == SEG_S1_COMP ? 1 : 2] += (u64)s.count;
+        }
+        a.counts[G] = ns > 0 ? frame_chunks(a, L) : 0;
+    }
+    // per-type position totals: one atomic per wave and type
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        u64 v = pos[k];
+        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(a.type_pos + k, (unsigned long long)v);
+    }
 }
 
 // Chunk j of c covers positions [j*L/c, (j+1)*L/c) of the concatenated

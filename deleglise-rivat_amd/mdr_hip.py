@@ -31,7 +31,7 @@ class Opts(ctypes.Structure):
                 ("fx", ctypes.c_double), ("u_cap_dense", ctypes.c_longlong), ("fx_set", ctypes.c_int),
                 ("u_cap_dense_set", ctypes.c_int), ("partition", ctypes.c_int), ("params", ctypes.c_int),
                 ("sched", ctypes.c_int), ("host_threads", ctypes.c_int), ("chunk", ctypes.c_longlong),
-                ("record_cycles", ctypes.c_int)]
+                ("record_cycles", ctypes.c_int), ("no_table_reuse", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -43,7 +43,7 @@ class Stats(ctypes.Structure):
                 ("mu_limit", ctypes.c_longlong), ("W", ctypes.c_longlong), ("u", ctypes.c_longlong),
                 ("N", ctypes.c_longlong), ("slots", ctypes.c_longlong), ("chunks", ctypes.c_longlong),
                 ("cx", ctypes.c_double), ("s1_terms", ctypes.c_longlong), ("s2_terms", ctypes.c_longlong),
-                ("table_bytes", ctypes.c_double)]
+                ("table_bytes", ctypes.c_double), ("s1_comp_terms", ctypes.c_longlong)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -103,12 +103,12 @@ def _i128(lo, hi):
 
 def make_opts(strategy=STRATEGY_AUTO, cx=0.0, u_cap=0, fx=0.0, u_cap_dense=0, fx_set=0,
               u_cap_dense_set=0, partition=PARTITION_STATIC, params=PARAMS_REFERENCE,
-              sched=SCHED_DYNAMIC, chunk=0, record_cycles=0):
+              sched=SCHED_DYNAMIC, chunk=0, record_cycles=0, no_table_reuse=0):
     o = Opts()
     load().mdr_default_opts(ctypes.byref(o))
     o.strategy, o.cx, o.u_cap, o.fx, o.u_cap_dense = strategy, cx, u_cap, fx, u_cap_dense
     o.fx_set, o.u_cap_dense_set, o.partition, o.params = fx_set, u_cap_dense_set, partition, params
-    o.sched, o.chunk, o.record_cycles = sched, chunk, record_cycles
+    o.sched, o.chunk, o.record_cycles, o.no_table_reuse = sched, chunk, record_cycles, no_table_reuse
     return o
 
 

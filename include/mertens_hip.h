@@ -71,6 +71,7 @@ typedef struct {
     int host_threads;         /* accepted for interface parity; the GPU path ignores it */
     long long chunk;          /* static chunk length in positions, 0 = autotuned default */
     int record_cycles;        /* 1: record per-chunk cycle counts (training data) */
+    int no_table_reuse;       /* 1: rebuild the table even when the cached one fits */
 } mdr_opts;
 
 typedef struct {
@@ -88,6 +89,7 @@ typedef struct {
     double cx;
     long long s1_terms, s2_terms;        /* quotient positions (sum over the call) */
     double table_bytes;                  /* bytes of the table arrays written */
+    long long s1_comp_terms;             /* S1 positions answered by the compressed tier */
 } mdr_stats;
 
 void mdr_default_opts(mdr_opts* opts);

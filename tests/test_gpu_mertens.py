@@ -1,0 +1,110 @@
+"""End-to-end M(X) on the GPU through the C ABI.
+
+Bit-exact against: the reference tests' known values (test_dr.cpp:26-47,
+test_cuda.cu:38-60, README.md:188), the compiled reference's outputs
+(tests/golden/ref_values.json), the oracle at random X, and -- at full sizes --
+the size-independent invariants of the method: M(X) does not depend on the
+cutoff u, the split cx, the strategy, the chunking, the schedule, the
+partition or on being evaluated alone or in a batch.
+"""
+import os
+import random
+import subprocess
+
+import pytest
+
+import mdr_hip
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def test_known_values_all_modes(engine, known):
+    modes = [dict(), dict(strategy=mdr_hip.STRATEGY_DENSE), dict(strategy=mdr_hip.STRATEGY_STREAMING),
+             dict(params=mdr_hip.PARAMS_TUNED), dict(sched=mdr_hip.SCHED_STATIC),
+             dict(partition=mdr_hip.PARTITION_LEARNED)]
+    for X, v in sorted(known.items()):
+        if X > 10**16:
+            continue
+        for kw in modes:
+            print("known", X, kw, flush=True)
+            assert engine.mertens(X, **kw) == v, (X, kw)
+
+
+def test_small_and_negative_inputs(engine):
+    for X in [-5, 0, 1, 2, 3, 4, 5]:
+        assert engine.mertens(X) == oracle.mertens(X, 1)
+    rng = random.Random(1)
+    xs = [rng.randrange(4, 5 * 10**7) for _ in range(40)] + [100000, 100001, 49999999, 50000000, 50000001]
+    for X in xs:
+        assert engine.mertens(X) == oracle.mertens(X, 8), X
+
+
+def test_reference_values(engine, ref_values):
+    # every value the compiled reference produced (up to ~1.3e16)
+    for X in sorted(ref_values):
+        assert engine.mertens(X) == ref_values[X], X
+
+
+def test_reference_values_as_one_batch(engine, ref_values):
+    xs = sorted(ref_values)
+    assert engine.batch(xs) == [ref_values[x] for x in xs]
+
+
+def test_oracle_random_points(engine):
+    rng = random.Random(2)
+    for _ in range(30):
+        X = rng.randrange(5 * 10**7, 10**12)
+        assert engine.mertens(X) == oracle.mertens(X, 8), X
+
+
+def test_compute_compressed_boundaries(engine):
+    # test_sieve.cpp:205-214 through the forced streaming strategy
+    f = mdr_hip.STRATEGY_FORCE_STREAMING
+    assert engine.mertens(99975680, strategy=f) == oracle.mertens_recurrence(99975680, 1 << 22)
+    assert engine.mertens(0, strategy=f) == 0
+    assert engine.mertens(10**8, strategy=f) == 1928
+    assert engine.mertens(10**10, strategy=f) == -33722
+
+
+def test_invariance_full_size(engine, known):
+    # M(1e16) under every control: u caps, cx, strategy, chunk length, schedule
+    X, v = 10**16, known[10**16]
+    for kw in [dict(cx=0.6), dict(cx=2.4), dict(u_cap=2 * 10**9), dict(u_cap=10**10),
+               dict(strategy=mdr_hip.STRATEGY_DENSE, u_cap_dense=4 * 10**9), dict(chunk=5000),
+               dict(chunk=1 << 20), dict(sched=mdr_hip.SCHED_STATIC, chunk=20000), dict(params=mdr_hip.PARAMS_TUNED)]:
+        print("invariance", kw, flush=True)
+        assert engine.mertens(X, **kw) == v, kw
+
+
+def test_mertens_1e17(engine, known):
+    assert engine.mertens(10**17) == known[10**17]
+    assert engine.mertens(10**17, params=mdr_hip.PARAMS_TUNED) == known[10**17]
+
+
+def test_batch_config_sample(engine, xlists, ref_values):
+    # batch workload: all values below 1e11 plus the 100 largest, checked
+    xs = xlists["batch_4096_1e9_1e14_seed3"]
+    sample = sorted(set([x for x in xs if x < 10**11] + sorted(xs)[-100:]))
+    got = engine.batch(sample)
+    assert got == [ref_values[x] for x in sample]
+
+
+def test_batch_equals_single(engine, xlists):
+    xs = xlists["batch_4096_1e9_1e14_seed3"][:64]
+    assert engine.batch(xs) == [engine.mertens(x) for x in xs]
+
+
+def test_large_batch_smallest_values(engine, xlists, ref_values):
+    xs = sorted(xlists["large_64_1e16_1e18_seed5"])[:4]
+    assert engine.batch(xs) == [ref_values[x] for x in xs]
+
+
+def test_dropin_cpp_program():
+    exe = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    r = subprocess.run([exe, str(10**16)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
