@@ -22,7 +22,7 @@ typedef unsigned long long u64;
 // Sieve segment: a multiple of 96 (so mu6 words align), of 64 (plane words)
 // and of 16 (aligned 16-byte stencil loads).
 constexpr int SEG = 49152;
-constexpr int SIEVE_THREADS = 256;
+constexpr int SIEVE_THREADS = 512;
 // Stencil period lcm(16, 4, 9, 25, 7, 11): weights of 2,3,5,7,11 and the square
 // flags of 4, 9 and 25 repeat with it; 16 | period keeps phases 16-byte aligned.
 constexpr int STENCIL_PERIOD = 277200;

@@ -245,7 +245,8 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
     c->n_w = (int)wp.size();
     c->n_small = (int)(std::lower_bound(wp.begin(), wp.end(), 256u) - wp.begin());
     c->n_sq = (int)sq.size();
-    c->n_sq_small = (int)(std::lower_bound(sq.begin(), sq.end(), 222u) - sq.begin());
+    // squares of p < 256 are flagged block-cooperatively; larger ones with the weights
+    c->n_sq_small = (int)(std::lower_bound(sq.begin(), sq.end(), 256u) - sq.begin());
     HIPT(c->wprimes.ensure(wp.size() * 4 + 4), "alloc primes");
     HIPT(c->wrcp.ensure(wr.size() * 8 + 8), "alloc primes");
     HIPT(c->wweights.ensure(ww.size() + 4), "alloc primes");
@@ -461,7 +462,8 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
              "scan counts");
         HIPT(c->recs.ensure((size_t)nrec * sizeof(ChunkRec)), "alloc chunk records");
         pa.recs = c->recs.as<ChunkRec>();
-        hipLaunchKernelGGL(k_expand, dim3(gslots), dim3(256), 0, c->st, pa);
+        pa.nrec = nrec;
+        hipLaunchKernelGGL(k_expand, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, c->st, pa);
         HIPT(hipGetLastError(), "launch expand");
 
         if (o.record_cycles) HIPT(c->stamps.ensure((size_t)nrec * 4), "alloc stamps");

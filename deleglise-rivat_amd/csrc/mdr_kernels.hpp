@@ -17,7 +17,7 @@ struct SieveArgs {
     const uint8_t* wweights;   // bit_width(p-1)|1
     int n_w, n_small;          // total, and those below 256
     const uint32_t* sqprimes;  // primes >= 7 whose squares are flagged
-    int n_sq, n_sq_small;      // total, and those below 222
+    int n_sq, n_sq_small;      // total, and those below 256 (larger ones go with the weights)
     int16_t* Md;
     u64* cpl;
     int32_t* cck;
@@ -44,6 +44,7 @@ struct PartArgs {
     i64* counts;               // per slot chunk count (k_count output)
     unsigned long long* type_pos;  // [3]: S1 dense, S1 compressed, S2 positions
     const i64* offsets;        // per slot first record (k_expand input)
+    i64 nrec;                  // total records
     ChunkRec* recs;
 };
 

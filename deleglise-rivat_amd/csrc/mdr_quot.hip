@@ -66,29 +66,36 @@ __global__ void k_count(PartArgs a) {
     }
 }
 
+// One thread per record: the record's slot is the last slot whose first
+// record is <= r (slots with no records share their successor's offset).
 // Chunk j of c covers positions [j*L/c, (j+1)*L/c) of the concatenated
 // segments; the first chunk also adds the G evaluation's constant.
 __global__ void k_expand(PartArgs a) {
-    const i64 G = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (G >= a.total_slots) return;
+    const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.nrec) return;
+    i64 lo = 0, hi = a.total_slots - 1;
+    while (lo < hi) {
+        const i64 mid = (lo + hi + 1) >> 1;
+        if (a.offsets[mid] <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    const i64 G = lo;
+    const i64 o = a.offsets[G];
+    const i64 c = (G + 1 < a.total_slots ? a.offsets[G + 1] : a.nrec) - o;
+    const i64 j = r - o;
     const int x = find_x(a.xp, a.nx, G);
     const XParam xp = a.xp[x];
     const i64 g = G - xp.slot_base;
     const Frame f = make_frame(xp, a.tp.mu6, g);
-    if (frame_nseg(f) == 0) return;
     const i64 L = frame_positions(f, xp, a.tp);
-    const i64 c = frame_chunks(a, L);
-    const i64 o = a.offsets[G];
-    for (i64 j = 0; j < c; ++j) {
-        ChunkRec r;
-        r.x = (uint32_t)x;
-        r.g = (uint32_t)g;
-        r.seg = j == 0 ? 0x100u : 0u;
-        r.pad = 0;
-        r.p_lo = (u64)((u128)j * (u64)L / (u64)c);
-        r.p_hi = (u64)((u128)(j + 1) * (u64)L / (u64)c);
-        a.recs[o + j] = r;
-    }
+    ChunkRec rec;
+    rec.x = (uint32_t)x;
+    rec.g = (uint32_t)g;
+    rec.seg = j == 0 ? 0x100u : 0u;
+    rec.pad = 0;
+    rec.p_lo = (u64)((u128)j * (u64)L / (u64)c);
+    rec.p_hi = (u64)((u128)(j + 1) * (u64)L / (u64)c);
+    a.recs[r] = rec;
 }
 
 // ---------------------------------------------------------------------------

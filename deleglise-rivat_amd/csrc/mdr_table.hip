@@ -75,7 +75,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
     __syncthreads();
 
-    // 2. square flags for p >= 7
+    // 2. square flags for 7 <= p < 256, block-cooperative over the multiples
     for (int i = 0; i < a.n_sq_small; ++i) {
         const uint32_t p = a.sqprimes[i];
         const uint32_t p2 = p * p;
@@ -83,32 +83,55 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
         for (uint32_t x = sq_off[i] + (uint32_t)tid * p2; x < (uint32_t)len; x += SIEVE_THREADS * p2)
             atomicOr(&acc32[x >> 2], 0x80u << (8 * (x & 3)));
     }
-    for (int i = a.n_sq_small + tid; i < a.n_sq; i += SIEVE_THREADS) {
-        const u64 p = a.sqprimes[i];
-        const u64 p2 = p * p;
-        if (p2 > lastn) break;
-        const u64 r = first % p2;
-        const u64 x = r ? p2 - r : 0;
-        if (x < (u64)len) atomicOr(&acc32[x >> 2], 0x80u << (8 * (x & 3)));
-    }
-    // 3. weights for p >= 13
+    // 3. weights for 13 <= p < 256, block-cooperative over the multiples
     for (int i = 0; i < a.n_small; ++i) {
         const uint32_t p = a.wprimes[i];
         const uint32_t w = a.wweights[i];
         for (uint32_t x = small_off[i] + (uint32_t)tid * p; x < (uint32_t)len; x += SIEVE_THREADS * p)
             atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
     }
-    for (int i = a.n_small + tid; i < a.n_w; i += SIEVE_THREADS) {
-        const uint32_t p = a.wprimes[i];
-        if ((u64)p * p > lastn) break;
-        const uint32_t w = a.wweights[i];
-        const uint32_t r = mod_u64(first, p, a.wrcp[i]);
-        for (uint32_t x = r ? p - r : 0; x < (uint32_t)len; x += p)
-            atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
+    // 4. primes 256 <= p <= sqrt(last n), one per thread, four in flight:
+    //    r2 = first mod p^2 gives the (at most one) square hit, r2 mod p the
+    //    first weight hit.  Primes ascend, so a thread stops at the first
+    //    p with p^2 beyond the segment.
+    constexpr int UNR = 4;
+    for (int i0 = a.n_small + tid; i0 < a.n_w; i0 += UNR * SIEVE_THREADS) {
+        uint32_t pp[UNR];
+        double rr[UNR];
+        uint32_t ww[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int i = i0 + u * SIEVE_THREADS;
+            const bool ok = i < a.n_w;
+            pp[u] = ok ? a.wprimes[i] : 0xffffffffu;
+            rr[u] = ok ? a.wrcp[i] : 0.0;
+            ww[u] = ok ? a.wweights[i] : 0u;
+        }
+        if ((u64)pp[0] * pp[0] > lastn) break;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const u64 p = pp[u];
+            const u64 p2 = p * p;
+            if (p2 > lastn) continue;
+            const double rp = rr[u];
+            i64 q2 = (i64)((double)first * (rp * rp));
+            i64 r2 = (i64)first - q2 * (i64)p2;
+            while (r2 < 0) r2 += (i64)p2;
+            while (r2 >= (i64)p2) r2 -= (i64)p2;
+            const u64 xs = r2 ? p2 - (u64)r2 : 0;
+            if (xs < (u64)len) atomicOr(&acc32[xs >> 2], 0x80u << (8 * (xs & 3)));
+            const i64 q1 = (i64)((double)r2 * rp);
+            i64 r = r2 - q1 * (i64)p;
+            if (r < 0) r += (i64)p;
+            else if (r >= (i64)p) r -= (i64)p;
+            const uint32_t w = ww[u];
+            for (uint32_t x = r ? (uint32_t)(p - r) : 0; x < (uint32_t)len; x += (uint32_t)p)
+                atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
+        }
     }
     __syncthreads();
 
-    // 4. finalise bytes to mu in {-1, 0, 1}
+    // 5. finalise bytes to mu in {-1, 0, 1}
     for (int i = tid; i < SEG / 4; i += SIEVE_THREADS) {
         uint32_t v = acc32[i];
         uint32_t out = 0;
@@ -127,7 +150,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
     __syncthreads();
 
-    // 5. group sums and compressed planes (ballots over 64 values)
+    // 6. group sums and compressed planes (ballots over 64 values)
     for (int g = wave; g < GROUPS; g += SIEVE_THREADS / 64) {
         const int8_t v = (int8_t)acc[g * 64 + lane];
         const u64 nz = __ballot(v != 0);
@@ -159,7 +182,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
     __syncthreads();
 
-    // 6. dense prefix values and compressed checkpoints (segment-local)
+    // 7. dense prefix values and compressed checkpoints (segment-local)
     for (int g = wave; g < GROUPS; g += SIEVE_THREADS / 64) {
         const u64 n0 = first + (u64)g * 64;
         if (n0 > (u64)a.U) break;
@@ -178,7 +201,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
         }
     }
 
-    // 7. packed mu for j coprime to 6 (t = j/3; 16 codes per word)
+    // 8. packed mu for j coprime to 6 (t = j/3; 16 codes per word)
     if (s < a.mu_segs) {
         const i64 w0 = s * (SEG / 48);
         for (int wi = tid; wi < SEG / 48; wi += SIEVE_THREADS) {
