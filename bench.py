@@ -103,8 +103,13 @@ def main():
     expect = known.get(X, ref.get(X))
 
     E = mdr_hip.Engine(0)
-    learned = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED, no_table_reuse=1)
-    static = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC, no_table_reuse=1)
+    # learned and static partitions under the same persistent static round-robin schedule
+    learned = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
+                                sched=mdr_hip.SCHED_STATIC, no_table_reuse=1)
+    static = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
+                               sched=mdr_hip.SCHED_STATIC, no_table_reuse=1)
+    dynamic = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
+                                sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
 
     def timed(opts, k, tag):
         res, stats = [], []
@@ -124,14 +129,17 @@ def main():
         E.mertens(X, learned)
         log(f"warmup {i} done")
     t_learned, st_l = timed(learned, args.steps, "learned")
-    t_static, st_s = (None, None)
+    used = "learned" if st_l[-1].partition_used == mdr_hip.PARTITION_LEARNED else "static (no cost model)"
+    t_static, st_s, t_dyn = None, None, None
     if not args.no_static:
         t_static, st_s = timed(static, args.steps, "static")
+        t_dyn, _ = timed(dynamic, args.steps, "static+queue")
 
     batch = None
     if not args.no_batch:
         xs = load_json("xlists.json")["batch_4096_1e9_1e14_seed3"]
-        bo = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED, no_table_reuse=1)
+        bo = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
+                               sched=mdr_hip.SCHED_STATIC, no_table_reuse=1)
         E.batch(xs[:64], bo)
         t0 = time.perf_counter()
         out = E.batch(xs, bo)
@@ -140,7 +148,8 @@ def main():
         checked = [i for i, x in enumerate(xs) if x in ref]
         bad = sum(out[i] != ref[xs[i]] for i in checked)
         one = xs[:256]
-        so = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED)
+        so = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
+                               sched=mdr_hip.SCHED_STATIC)
         t0 = time.perf_counter()
         single = [E.mertens(x, so) for x in one]
         t1 = time.perf_counter() - t0
@@ -165,13 +174,15 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_learned * 1000, 2),
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "int64",
         "data": "synthetic",
-        "config": {"workload": f"single exact M({X:.0e}) incl. table build, learned partition", "X": X,
+        "config": {"workload": f"single exact M({X:.0e}) incl. table build, {used} partition", "X": X,
                    "U": st.U, "u_dense": st.u_dense, "cx": st.cx, "N": st.N, "chunks": st.chunks,
                    "parallelism": "1 GPU"},
         "result": E.mertens(X, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED)) if expect is None else expect,
         "wall_s_learned": round(t_learned, 4),
         "wall_s_static": None if t_static is None else round(t_static, 4),
         "static_over_learned": None if t_static is None else round(t_static / t_learned, 3),
+        "wall_s_static_dynamic_queue": None if t_dyn is None else round(t_dyn, 4),
+        "partition_used": used,
         "phase_ms": {"table": round(st.table_ms, 2), "partition": round(st.partition_ms, 2),
                      "quotient": round(st.quotient_ms, 2)},
         "quotient_positions": {"s1_dense": st.s1_terms - st.s1_comp_terms, "s1_comp": st.s1_comp_terms,

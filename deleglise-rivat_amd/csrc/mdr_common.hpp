@@ -2,13 +2,15 @@
 //
 // HBM layout of one table (all arrays device-resident, built by the sieve):
 //   Md    int16 [ud + 1]         M(n) for n <= ud (dense tier)
-//   cpl   u64x2 [W]              per 64-value word above ud: {nonzero, negative}
-//                                bit planes (word w covers n = ud+1+64w .. ud+64+64w)
-//   cck   int32 [W]              M(ud + 64w), the value just before word w
+//   cl    CLine [L]              compressed tier above ud: one 64-byte line per
+//                                192 values (three 64-value words); word w of a
+//                                line holds the nonzero and negative bit planes
+//                                of its values and M just before its first value
 //   mu6   u32   [ceil(T6/16)]    2-bit codes mu(j)+1 for j coprime to 6, indexed
 //                                by t = floor(j/3), so j(t) = 3t + 1 + (t & 1)
-// The reference keeps the same two tiers (mertens_dr.hpp:662-752) with the
-// planes in 256-value chunks; here a word's planes are one 16-byte load.
+// The reference keeps the same two tiers (mertens_dr.hpp:662-752) but stores
+// the planes in 256-value chunks and the checkpoints in a separate array; here
+// a scattered lookup touches exactly one 64-byte line.
 #ifndef MDR_COMMON_HPP
 #define MDR_COMMON_HPP
 
@@ -26,6 +28,15 @@ constexpr int SIEVE_THREADS = 512;
 // Stencil period lcm(16, 4, 9, 25, 7, 11): weights of 2,3,5,7,11 and the square
 // flags of 4, 9 and 25 repeat with it; 16 | period keeps phases 16-byte aligned.
 constexpr int STENCIL_PERIOD = 277200;
+// Values per compressed line; u_dense must be a multiple of it below U, and
+// SEG is a multiple of it, so lines never straddle segments.
+constexpr int LINE_VALUES = 192;
+
+struct alignas(64) CLine {
+    u64 pl[6];        // word w: pl[2w] = nonzero plane, pl[2w+1] = negative plane
+    int32_t ck[3];    // word w: M(value before its first value)
+    int32_t pad;
+};
 
 // Segment kinds of one G evaluation.
 enum SegType : uint8_t {
@@ -37,14 +48,16 @@ enum SegType : uint8_t {
 
 constexpr int MAX_SEGS = 12;
 
-// One quotient-phase work chunk: positions [p_lo, p_hi) of one segment.
-struct ChunkRec {
-    uint32_t x;        // index of X in the batch
-    uint32_t g;        // G-evaluation slot within that X
-    uint32_t seg;      // segment index (0..MAX_SEGS-1); bit 8 = add the constant
-    uint32_t pad;
-    u64 p_lo;
-    u64 p_hi;
+// One record of the cycle log (training data for the cost model): the
+// positions of one cost piece processed by one wave and the shader-clock
+// cycles they took.
+struct StampRec {
+    i64 y;          // argument of the G evaluation
+    i64 start;      // first n (S1) or j (S2)
+    i64 end;        // last n or j
+    uint32_t kind;  // cost-model kind (0..15)
+    uint32_t cycles;
+    u64 count;      // positions
 };
 
 // Per-X parameters (device-resident array).
@@ -61,12 +74,11 @@ struct XParam {
 // Table parameters (kernel argument).
 struct TableParam {
     i64 U;            // table covers n = 1..U
-    i64 ud;           // dense tier limit (ud % 64 == 0 when W > 0)
-    i64 W;            // compressed words
+    i64 ud;           // dense tier limit (ud % LINE_VALUES == 0 when L > 0)
+    i64 L;            // compressed lines
     i64 mu_lim;       // mu6 covers j <= mu_lim
     const int16_t* Md;
-    const u64* cpl;   // 2 u64 per word
-    const int32_t* cck;
+    const CLine* cl;
     const uint32_t* mu6;
 };
 

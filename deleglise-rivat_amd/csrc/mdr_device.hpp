@@ -103,14 +103,21 @@ __device__ __forceinline__ int mu6_code(const uint32_t* mu6, i64 t) {
     return (int)((mu6[t >> 4] >> (2 * (t & 15))) & 3u);
 }
 
+// M(q) in the compressed tier (q > ud): the value's line, word and bit.
+__device__ __forceinline__ int32_t comp_M(const CLine* cl, u64 off) {
+    const uint32_t w = (uint32_t)(off >> 6);     // off < 2^38
+    const uint32_t line = w / 3u;
+    const uint32_t sub = w - 3u * line;
+    const CLine* L = cl + line;
+    const u64 nz = L->pl[2 * sub], ng = L->pl[2 * sub + 1];
+    const u64 mask = ~0ull >> (63 - (off & 63));
+    return L->ck[sub] + (int)__popcll(nz & mask) - 2 * (int)__popcll(ng & mask);
+}
+
 // M(q) from either tier (mertens_dr.hpp:707-712, :741-752).
 __device__ __forceinline__ int32_t get_M(const TableParam& tp, i64 q) {
     if (q <= tp.ud) return tp.Md[q];
-    const i64 off = q - tp.ud - 1;
-    const i64 w = off >> 6;
-    const u64 nz = tp.cpl[2 * w], ng = tp.cpl[2 * w + 1];
-    const u64 mask = ~0ull >> (63 - (off & 63));
-    return tp.cck[w] + (int)__popcll(nz & mask) - 2 * (int)__popcll(ng & mask);
+    return comp_M(tp.cl, (u64)(q - tp.ud - 1));
 }
 
 // ---------------------------------------------------------------------------

@@ -18,9 +18,16 @@
 #include "mertens_hip.h"
 #include "mdr_device.hpp"
 #include "mdr_kernels.hpp"
+#include "mdr_mlp.hpp"
 #include "mdr_tune.hpp"
 
+#include <dlfcn.h>
+#include <random>
+
 using namespace mdr;
+
+static_assert(sizeof(mdr_stamp) == sizeof(StampRec), "cycle-log record layout");
+static_assert(MDR_MLP_NPARAM == MLP_NPARAM && MDR_MLP_NFEAT == MLP_IN, "cost-model size");
 
 namespace {
 
@@ -73,28 +80,28 @@ struct mdr_ctx {
 
     // table
     bool tab_valid = false;
-    i64 tU = 0, tud = 0, tmu = 0, tW = 0, tnseg = 0, tmusegs = 0;
-    DevBuf Md, cpl, cck, mu6, segsum, scan_tmp;
+    i64 tU = 0, tud = 0, tmu = 0, tL = 0, tnseg = 0, tmusegs = 0;
+    DevBuf Md, cl, mu6, segsum, scan_tmp;
     // sieving primes
     i64 primes_root = -1;
     int n_w = 0, n_small = 0, n_sq = 0, n_sq_small = 0;
     DevBuf stencil, wprimes, wrcp, wweights, sqprimes;
     // work
-    DevBuf xparam, counts, recs, out128, ctr, stamps, comp_out;
-    i64 last_nrec = 0;
-    std::vector<uint32_t> last_stamps;
-    std::vector<ChunkRec> last_recs;
-    std::vector<XParam> last_xp;
+    DevBuf xparam, slot_len, slot_cost, bounds, out128, ctr, stamps, comp_out;
+    std::vector<StampRec> last_stamps;
+    // learned cost model
+    DevBuf mlp;
+    bool mlp_ok = false;
+    std::vector<float> mlp_host;
 
     TableParam tp() const {
         TableParam t;
         t.U = tU;
         t.ud = tud;
-        t.W = tW;
+        t.L = tL;
         t.mu_lim = tmu;
         t.Md = Md.as<int16_t>();
-        t.cpl = cpl.as<u64>();
-        t.cck = cck.as<int32_t>();
+        t.cl = cl.as<CLine>();
         t.mu6 = mu6.as<uint32_t>();
         return t;
     }
@@ -176,6 +183,9 @@ mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
     if (o.params < 0 || o.params > 1) return fail(c, MDR_ERR_INVALID, "params out of range");
     if (o.sched < 0 || o.sched > 1) return fail(c, MDR_ERR_INVALID, "sched out of range");
     if (o.chunk < 0) return fail(c, MDR_ERR_INVALID, "chunk must be >= 0");
+    if (o.tune_u < 0 || o.tune_ud < 0 || o.tune_cx < 0 || (o.tune_cx != 0 && !(o.tune_cx > 0.1 && o.tune_cx < 5.0)))
+        return fail(c, MDR_ERR_INVALID, "tuning override out of range");
+    if (o.tune_u > 1000000000000LL) return fail(c, MDR_ERR_INVALID, "tune_u above 1e12");
     return MDR_OK;
 }
 
@@ -199,11 +209,15 @@ XPlan plan_x(i64 X, const mdr_opts& o) {
     p.cx = cx;
     const i64 A_max = (i64)(cx * std::sqrt((double)X)) + 1000;
     if (o.params == MDR_PARAMS_TUNED && X >= 100000000LL) {
-        const TuneChoice t = tune_lookup(X, streaming);
+        TuneChoice t = tune_lookup(X, streaming);
+        if (o.tune_cx > 0) t.cx = o.tune_cx;
+        if (o.tune_u > 0) t.u = std::max<i64>(((o.tune_u + LINE_VALUES - 1) / LINE_VALUES) * LINE_VALUES,
+                                              ((i64)(t.cx * std::sqrt((double)X)) + 1000 + LINE_VALUES) / LINE_VALUES * LINE_VALUES);
+        if (o.tune_ud > 0) t.ud = std::min(t.u, (o.tune_ud / LINE_VALUES) * LINE_VALUES);
         p.cx = t.cx;
         const i64 A2 = (i64)(p.cx * std::sqrt((double)X)) + 1000;
         p.u = t.u;
-        p.ud = streaming ? std::min(p.u, std::max<i64>(t.ud, ((A2 + 63) / 64) * 64)) : p.u;
+        p.ud = std::min(p.u, t.ud);
         p.N = X / p.u;
         p.mu_lim = std::max(A2, p.N + 1000);
     } else if (streaming) {
@@ -280,7 +294,8 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
 mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, bool reuse = true) {
     if (U < 1) U = 1;
     if (ud > U) ud = U;
-    if (ud < U && (ud % 64) != 0) return fail(c, MDR_ERR_INVALID, "u_dense must be a multiple of 64 below U");
+    if (ud < U && (ud % LINE_VALUES) != 0)
+        return fail(c, MDR_ERR_INVALID, "u_dense must be a multiple of 192 below U");
     if (mu_lim > U) return fail(c, MDR_ERR_INVALID, "mu_limit above U");
     if (mu_lim < 1) mu_lim = 1;
     if (reuse && c->tab_valid && c->tU == U && c->tud == ud && c->tmu == mu_lim) {
@@ -292,10 +307,9 @@ mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, 
     if (s != MDR_OK) return s;
     const i64 nseg = (U + SEG - 1) / SEG;
     const i64 musegs = (mu_lim + SEG - 1) / SEG;
-    const i64 W = U > ud ? (U - ud + 63) / 64 : 0;
+    const i64 NL = U > ud ? (U - ud + LINE_VALUES - 1) / LINE_VALUES : 0;
     HIPT(c->Md.ensure((size_t)(ud + 1) * 2), "alloc dense table");
-    HIPT(c->cpl.ensure((size_t)W * 16), "alloc planes");
-    HIPT(c->cck.ensure((size_t)W * 4), "alloc checkpoints");
+    HIPT(c->cl.ensure((size_t)NL * sizeof(CLine)), "alloc compressed lines");
     HIPT(c->mu6.ensure((size_t)musegs * (SEG / 48) * 4), "alloc mu");
     HIPT(c->segsum.ensure((size_t)nseg * 8), "alloc segment sums");
     HIPT(c->scan_tmp.ensure((size_t)scan_tmp_len(nseg) * 8), "alloc scan");
@@ -304,7 +318,7 @@ mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, 
     SieveArgs a;
     a.U = U;
     a.ud = ud;
-    a.W = W;
+    a.L = NL;
     a.mu_segs = musegs;
     a.stencil = c->stencil.as<uint8_t>();
     a.wprimes = c->wprimes.as<uint32_t>();
@@ -316,8 +330,7 @@ mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, 
     a.n_sq = c->n_sq;
     a.n_sq_small = c->n_sq_small;
     a.Md = c->Md.as<int16_t>();
-    a.cpl = c->cpl.as<u64>();
-    a.cck = c->cck.as<int32_t>();
+    a.cl = c->cl.as<CLine>();
     a.mu6 = c->mu6.as<uint32_t>();
     a.segsum = c->segsum.as<i64>();
     const i64 maxgrid = 1 << 30;
@@ -332,11 +345,11 @@ mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, 
          "scan segments");
     FixupArgs f;
     f.ud = ud;
-    f.W = W;
+    f.L = NL;
     f.Md = c->Md.as<int16_t>();
-    f.cck = c->cck.as<int32_t>();
+    f.cl = c->cl.as<CLine>();
     f.segoff = c->segsum.as<i64>();
-    const i64 nfix = std::max(ud, W);
+    const i64 nfix = std::max(ud, NL);
     if (nfix > 0) hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)((nfix + 255) / 256)), dim3(256), 0, c->st, f);
     HIPT(hipGetLastError(), "launch fixup");
     HIPT(hipEventRecord(c->ev[1], c->st), "event");
@@ -344,7 +357,7 @@ mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, 
     c->tU = U;
     c->tud = ud;
     c->tmu = mu_lim;
-    c->tW = W;
+    c->tL = NL;
     c->tnseg = nseg;
     c->tmusegs = musegs;
     c->tab_valid = true;
@@ -352,7 +365,7 @@ mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, 
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
         stats->table_ms += ms;
-        stats->table_bytes += 2.0 * (double)(ud + 1) + 20.0 * (double)W + (double)musegs * (SEG / 12);
+        stats->table_bytes += 2.0 * (double)(ud + 1) + (double)sizeof(CLine) * (double)NL + (double)musegs * (SEG / 12);
     }
     return MDR_OK;
 }
@@ -370,20 +383,21 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
                      mdr_stats* stats) {
     const size_t nx = plans.size();
     // table covering every X
-    i64 U = 1, mu_lim = 1, ud_stream = 0;
-    bool any_stream = false;
+    i64 U = 1, mu_lim = 1, ud = 0;
+    bool any_sum = false;
     for (const XPlan& p : plans) {
         if (p.kind == 0) continue;
         U = std::max(U, std::max(p.u, p.mu_lim));
         if (p.kind == 1) U = std::max(U, p.X);
         mu_lim = std::max(mu_lim, p.mu_lim);
-        if (p.kind == 2 && p.streaming) {
-            any_stream = true;
-            ud_stream = std::max(ud_stream, p.ud);
+        if (p.kind == 2) {
+            any_sum = true;
+            ud = std::max(ud, p.ud);
         }
     }
-    i64 ud = any_stream ? std::min(U, ud_stream) : U;
-    if (ud < U) ud = (ud / 64) * 64;
+    // one dense tier for the batch: the largest any X asked for
+    ud = any_sum ? std::min(U, ud) : U;
+    if (ud < U) ud = (ud / LINE_VALUES) * LINE_VALUES;
     const auto t_call = std::chrono::steady_clock::now();
     bool need_table = false;
     for (const XPlan& p : plans) need_table |= p.kind != 0;
@@ -429,55 +443,73 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
                            c->out128.as<u64>());
         HIPT(hipGetLastError(), "launch lookup");
     }
-    i64 nrec = 0;
+    i64 P = 0;
     if (!xps.empty()) {
-        // xparam.x holds the batch index of each X for the result slot
-        std::vector<XParam> upload = xps;
-        HIPT(c->xparam.ensure(upload.size() * sizeof(XParam)), "alloc params");
+        HIPT(c->xparam.ensure(xps.size() * sizeof(XParam)), "alloc params");
         const auto th0 = std::chrono::steady_clock::now();
-        HIPT(hipMemcpyAsync(c->xparam.p, upload.data(), upload.size() * sizeof(XParam), hipMemcpyHostToDevice,
-                            c->st),
+        HIPT(hipMemcpyAsync(c->xparam.p, xps.data(), xps.size() * sizeof(XParam), hipMemcpyHostToDevice, c->st),
              "upload params");
         HIPT(hipStreamSynchronize(c->st), "upload params");
         if (stats) stats->h2d_copy_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
-        HIPT(c->counts.ensure((size_t)slots * 8), "alloc counts");
+        const bool learned = o.partition == MDR_PARTITION_LEARNED && c->mlp_ok;
+        const i64 nwaves = (i64)c->ncu * c->quot_blocks_per_cu * 4;
+        i64 tuned_chunks = 0;
+        if (o.params == MDR_PARAMS_TUNED) tuned_chunks = tune_chunks(plans.empty() ? 1 : plans.back().X);
+        P = o.chunk > 0 ? o.chunk : tuned_chunks > 0 ? tuned_chunks : nwaves * (o.sched == MDR_SCHED_DYNAMIC ? 16 : 4);
+        HIPT(c->slot_len.ensure((size_t)slots * 8), "alloc slot lengths");
+        if (learned) HIPT(c->slot_cost.ensure((size_t)slots * 8), "alloc slot costs");
+        HIPT(c->bounds.ensure((size_t)(P + 1) * 8), "alloc chunk bounds");
         HIPT(c->scan_tmp.ensure((size_t)std::max(scan_tmp_len(slots), scan_tmp_len(c->tnseg)) * 8), "alloc scan");
+        HIPT(c->ctr.ensure(64), "alloc counters");
+        HIPT(hipMemsetAsync(c->ctr.p, 0, 64, c->st), "clear counters");
+        // ctr layout: [0] work queue, [1..3] position counts, [4] total positions,
+        // [5] total predicted cycles, [6] cycle-log count
+        unsigned long long* ctr = c->ctr.as<unsigned long long>();
         PartArgs pa{};
         pa.xp = c->xparam.as<XParam>();
         pa.nx = (int)xps.size();
         pa.total_slots = slots;
         pa.tp = tp;
-        pa.chunk = o.chunk > 0 ? o.chunk : tune_chunk();
-        pa.counts = c->counts.as<i64>();
-        pa.offsets = c->counts.as<i64>();
-        HIPT(c->ctr.ensure(64), "alloc counter");
-        HIPT(hipMemsetAsync(c->ctr.p, 0, 64, c->st), "clear counters");
-        pa.type_pos = c->ctr.as<unsigned long long>() + 1;
+        pa.mlp = learned ? c->mlp.as<float>() : nullptr;
+        pa.len = c->slot_len.as<i64>();
+        pa.cost = learned ? c->slot_cost.as<i64>() : nullptr;
+        pa.type_pos = ctr + 1;
+        pa.totals = reinterpret_cast<const i64*>(ctr + 4);
+        pa.P = P;
+        pa.bounds = c->bounds.as<i64>();
         HIPT(hipEventRecord(c->ev[2], c->st), "event");
         const unsigned gslots = (unsigned)((slots + 255) / 256);
-        hipLaunchKernelGGL(k_count, dim3(gslots), dim3(256), 0, c->st, pa);
-        HIPT(hipGetLastError(), "launch count");
-        HIPT(scan_exclusive_i64(c->counts.as<i64>(), c->counts.as<i64>(), slots, c->scan_tmp.as<i64>(),
-                                scan_tmp_len(slots), &nrec, c->st),
-             "scan counts");
-        HIPT(c->recs.ensure((size_t)nrec * sizeof(ChunkRec)), "alloc chunk records");
-        pa.recs = c->recs.as<ChunkRec>();
-        pa.nrec = nrec;
-        hipLaunchKernelGGL(k_expand, dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, c->st, pa);
-        HIPT(hipGetLastError(), "launch expand");
-
-        if (o.record_cycles) HIPT(c->stamps.ensure((size_t)nrec * 4), "alloc stamps");
-        // records index X by its position in xps; the per-X sums land in
-        // comp_out and are remapped to batch positions below
+        hipLaunchKernelGGL(k_slot_len, dim3(gslots), dim3(256), 0, c->st, pa);
+        HIPT(hipGetLastError(), "launch slot lengths");
+        const i64 nt = (slots + 2047) / 2048;
+        HIPT(scan_exclusive_i64(pa.len, pa.len, slots, c->scan_tmp.as<i64>(), scan_tmp_len(slots), nullptr, c->st),
+             "scan positions");
+        HIPT(hipMemcpyAsync(ctr + 4, c->scan_tmp.as<i64>() + nt, 8, hipMemcpyDeviceToDevice, c->st), "total");
+        if (learned) {
+            HIPT(scan_exclusive_i64(pa.cost, pa.cost, slots, c->scan_tmp.as<i64>(), scan_tmp_len(slots), nullptr,
+                                    c->st),
+                 "scan costs");
+            HIPT(hipMemcpyAsync(ctr + 5, c->scan_tmp.as<i64>() + nt, 8, hipMemcpyDeviceToDevice, c->st), "total");
+        }
+        hipLaunchKernelGGL(k_cuts, dim3((unsigned)((P + 1 + 255) / 256)), dim3(256), 0, c->st, pa);
+        HIPT(hipGetLastError(), "launch cuts");
+        // stamps (training data of the cost model)
+        const i64 cap = o.record_cycles ? (i64)1 << 22 : 0;
+        if (cap) HIPT(c->stamps.ensure((size_t)cap * sizeof(StampRec)), "alloc cycle log");
         QuotArgs qa{};
-        qa.xp = c->xparam.as<XParam>();
+        qa.xp = pa.xp;
+        qa.nx = pa.nx;
         qa.tp = tp;
-        qa.recs = c->recs.as<ChunkRec>();
-        qa.nrec = nrec;
-        qa.next = c->ctr.as<unsigned long long>();
-        qa.stamps = o.record_cycles ? c->stamps.as<uint32_t>() : nullptr;
+        qa.off = pa.len;
+        qa.nslots = slots;
+        qa.bounds = pa.bounds;
+        qa.P = P;
+        qa.next = ctr;
         qa.dynamic = o.sched == MDR_SCHED_DYNAMIC ? 1 : 0;
-        static_assert(sizeof(ChunkRec) == 32, "record layout");
+        qa.stamps = cap ? c->stamps.as<StampRec>() : nullptr;
+        qa.nstamps = ctr + 6;
+        qa.stamp_cap = cap;
+        qa.stamp_stride = o.record_cycles > 1 ? o.record_cycles : 1;
         HIPT(c->comp_out.ensure(xps.size() * 16 + 16), "alloc accumulators");
         HIPT(hipMemsetAsync(c->comp_out.p, 0, xps.size() * 16, c->st), "clear accumulators");
         qa.out128 = c->comp_out.as<u64>();
@@ -501,27 +533,26 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
             if (hi != (lo >> 63)) return fail(c, MDR_ERR_RANGE, "result does not fit in 64 bits");
             out[i] = lo;
         }
+        unsigned long long ch[7];
+        HIPT(hipMemcpy(ch, c->ctr.p, sizeof ch, hipMemcpyDeviceToHost), "download");
         if (stats) {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
             stats->partition_ms += ms;
             (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]);
             stats->quotient_ms += ms;
-            stats->chunks += nrec;
+            stats->chunks += P;
             stats->slots += slots;
-            unsigned long long tp3[3];
-            HIPT(hipMemcpy(tp3, c->ctr.as<unsigned long long>() + 1, sizeof tp3, hipMemcpyDeviceToHost), "download");
-            stats->s1_terms += (long long)(tp3[0] + tp3[1]);
-            stats->s1_comp_terms += (long long)tp3[1];
-            stats->s2_terms += (long long)tp3[2];
+            stats->s1_terms += (long long)(ch[1] + ch[2]);
+            stats->s1_comp_terms += (long long)ch[2];
+            stats->s2_terms += (long long)ch[3];
+            stats->partition_used = learned ? MDR_PARTITION_LEARNED : MDR_PARTITION_STATIC;
         }
-        if (o.record_cycles) {
-            c->last_stamps.resize((size_t)nrec);
-            c->last_recs.resize((size_t)nrec);
-            HIPT(hipMemcpy(c->last_stamps.data(), c->stamps.p, (size_t)nrec * 4, hipMemcpyDeviceToHost), "download");
-            HIPT(hipMemcpy(c->last_recs.data(), c->recs.p, (size_t)nrec * sizeof(ChunkRec), hipMemcpyDeviceToHost),
-                 "download");
-            c->last_xp = xps;
+        if (cap) {
+            const i64 n = std::min<i64>((i64)ch[6], cap);
+            c->last_stamps.resize((size_t)n);
+            if (n) HIPT(hipMemcpy(c->last_stamps.data(), c->stamps.p, (size_t)n * sizeof(StampRec), hipMemcpyDeviceToHost),
+                        "download cycle log");
         }
     } else {
         HIPT(hipStreamSynchronize(c->st), "lookups");
@@ -529,12 +560,11 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
         HIPT(hipMemcpy(res.data(), c->out128.p, nx * 16, hipMemcpyDeviceToHost), "download");
         for (size_t i = 0; i < nx; ++i) out[i] = plans[i].kind == 0 ? plans[i].value : (i64)res[2 * i];
     }
-    c->last_nrec = nrec;
     if (stats) {
         stats->U = c->tU;
         stats->u_dense = c->tud;
         stats->mu_limit = c->tmu;
-        stats->W = c->tW;
+        stats->W = c->tL;
         if (!plans.empty()) {
             const XPlan& p = plans.back();
             stats->u = p.u;
@@ -580,6 +610,19 @@ mdr_status mdr_init(mdr_ctx** out, int device) {
         return MDR_ERR_HIP;
     }
     c->quot_blocks_per_cu = std::max(1, occ);
+    // the trained cost model: MDR_COST_MODEL, else cost_model.bin beside the library
+    std::string path;
+    if (const char* e = std::getenv("MDR_COST_MODEL")) path = e;
+    else {
+        Dl_info info;
+        if (dladdr((void*)&mdr_init, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/cost_model.bin";
+        }
+    }
+    if (!path.empty()) (void)mdr_cost_model_load(c, path.c_str());   // absent or invalid: static partition
+    c->err.clear();
     *out = c;
     return MDR_OK;
 }
@@ -588,9 +631,9 @@ void mdr_shutdown(mdr_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
-    DevBuf* bufs[] = {&c->Md, &c->cpl, &c->cck, &c->mu6, &c->segsum, &c->scan_tmp, &c->stencil, &c->wprimes,
-                      &c->wrcp, &c->wweights, &c->sqprimes, &c->xparam, &c->counts, &c->recs, &c->out128,
-                      &c->ctr, &c->stamps, &c->comp_out};
+    DevBuf* bufs[] = {&c->Md, &c->cl, &c->mu6, &c->segsum, &c->scan_tmp, &c->stencil, &c->wprimes,
+                      &c->wrcp, &c->wweights, &c->sqprimes, &c->xparam, &c->slot_len, &c->slot_cost, &c->bounds,
+                      &c->out128, &c->ctr, &c->stamps, &c->comp_out, &c->mlp};
     for (DevBuf* b : bufs) b->release();
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -827,7 +870,7 @@ mdr_status mdr_s1_sum(mdr_ctx* c, long long y, long long first, long long last, 
             if (stride == 2 && ((a - first) & 1)) ++a;
         }
         if (a > b) continue;
-        if (comp && c->tW == 0) return fail(c, MDR_ERR_INVALID, "quotient beyond the dense table");
+        if (comp && c->tL == 0) return fail(c, MDR_ERR_INVALID, "quotient beyond the dense table");
         sg.first = a;
         sg.count = (b - a) / stride + 1;
         long long l, h;
@@ -847,6 +890,205 @@ mdr_status mdr_g_eval(mdr_ctx* c, long long y, int comb2, long long u, double cx
     if (u > c->tU) return fail(c, MDR_ERR_INVALID, "u beyond the table");
     if ((i64)(cx * std::sqrt((double)y)) > c->tmu) return fail(c, MDR_ERR_INVALID, "split point beyond the mu table");
     return run_component(c, Seg{}, y, 1, comb2 ? 1 : 0, u, cx, lo, hi);
+}
+
+}  // extern "C"
+
+// ---- learned cost model --------------------------------------------------------------
+namespace {
+const char MODEL_MAGIC[8] = {'M', 'D', 'R', 'M', 'L', 'P', '0', '1'};
+
+mdr_status set_model(mdr_ctx* c, const float* w, int n) {
+    if (n == 0) {
+        c->mlp_ok = false;
+        c->mlp_host.clear();
+        return MDR_OK;
+    }
+    if (n != MLP_NPARAM || !w) return fail(c, MDR_ERR_INVALID, "cost model needs MDR_MLP_NPARAM weights");
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(w[i])) return fail(c, MDR_ERR_INVALID, "cost model weights must be finite");
+    HIPT(c->mlp.ensure(sizeof(float) * n), "alloc cost model");
+    HIPT(hipMemcpy(c->mlp.p, w, sizeof(float) * n, hipMemcpyHostToDevice), "upload cost model");
+    c->mlp_host.assign(w, w + n);
+    c->mlp_ok = true;
+    return MDR_OK;
+}
+
+struct DevTmp {
+    void* p = nullptr;
+    ~DevTmp() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t b) { return hipMalloc(&p, b < 16 ? 16 : b); }
+    template <typename T> T* as() { return static_cast<T*>(p); }
+};
+}  // namespace
+
+extern "C" {
+
+mdr_status mdr_cost_model_set(mdr_ctx* c, const float* w, int n) {
+    if (!c) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return set_model(c, w, n);
+}
+
+mdr_status mdr_cost_model_get(mdr_ctx* c, float* w, int n) {
+    if (!c || !w) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->mlp_ok) return fail(c, MDR_ERR_INVALID, "no cost model");
+    if (n < MLP_NPARAM) return fail(c, MDR_ERR_INVALID, "buffer too small");
+    std::memcpy(w, c->mlp_host.data(), sizeof(float) * MLP_NPARAM);
+    return MDR_OK;
+}
+
+mdr_status mdr_cost_model_load(mdr_ctx* c, const char* path) {
+    if (!c || !path) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(c, MDR_ERR_INVALID, std::string("cannot open ") + path);
+    char magic[8];
+    std::vector<float> w(MLP_NPARAM);
+    const bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, MODEL_MAGIC, 8) == 0 &&
+                    std::fread(w.data(), sizeof(float), MLP_NPARAM, f) == (size_t)MLP_NPARAM;
+    std::fclose(f);
+    if (!ok) return fail(c, MDR_ERR_INVALID, std::string("not a cost model file: ") + path);
+    return set_model(c, w.data(), MLP_NPARAM);
+}
+
+mdr_status mdr_cost_model_save(mdr_ctx* c, const char* path) {
+    if (!c || !path) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->mlp_ok) return fail(c, MDR_ERR_INVALID, "no cost model");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(c, MDR_ERR_INVALID, std::string("cannot write ") + path);
+    const bool ok = std::fwrite(MODEL_MAGIC, 1, 8, f) == 8 &&
+                    std::fwrite(c->mlp_host.data(), sizeof(float), MLP_NPARAM, f) == (size_t)MLP_NPARAM;
+    std::fclose(f);
+    return ok ? MDR_OK : fail(c, MDR_ERR_INVALID, "short write");
+}
+
+mdr_status mdr_cycle_log(mdr_ctx* c, mdr_stamp* out, size_t cap, size_t* n) {
+    if (!c || !n || (cap && !out)) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t k = std::min(cap, c->last_stamps.size());
+    if (k) std::memcpy(out, c->last_stamps.data(), k * sizeof(mdr_stamp));
+    *n = c->last_stamps.size();
+    return MDR_OK;
+}
+
+mdr_status mdr_mlp_features(const mdr_stamp* s, size_t n, float* feats, float* targets) {
+    if ((n && (!s || !feats || !targets))) return MDR_ERR_INVALID;
+    for (size_t i = 0; i < n; ++i) {
+        if (s[i].y < 1 || s[i].start < 1 || s[i].end < s[i].start || s[i].count == 0) return MDR_ERR_INVALID;
+        piece_features(feats + i * MLP_IN, s[i].y, s[i].start, s[i].end, (int)s[i].kind);
+        targets[i] = std::log2((float)(s[i].cycles > 0 ? s[i].cycles : 1) / (float)s[i].count);
+    }
+    return MDR_OK;
+}
+
+mdr_status mdr_mlp_predict(mdr_ctx* c, const float* w, const float* feats, size_t n, float* out) {
+    if (!c || !w || (n && (!feats || !out))) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (n == 0) return MDR_OK;
+    DevTmp dw, dx, dy;
+    HIPT(dw.alloc(sizeof(float) * MLP_NPARAM), "alloc");
+    HIPT(dx.alloc(sizeof(float) * MLP_IN * n), "alloc");
+    HIPT(dy.alloc(sizeof(float) * n), "alloc");
+    HIPT(hipMemcpy(dw.p, w, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice), "upload");
+    HIPT(hipMemcpy(dx.p, feats, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice), "upload");
+    HIPT(mlp_predict(dw.as<float>(), dx.as<float>(), (i64)n, dy.as<float>(), c->st), "predict");
+    HIPT(hipMemcpyAsync(out, dy.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->st), "download");
+    HIPT(hipStreamSynchronize(c->st), "predict");
+    return MDR_OK;
+}
+
+mdr_status mdr_mlp_grad(mdr_ctx* c, const float* w, const float* feats, const float* targets, size_t n, float* grad,
+                        float* loss) {
+    if (!c || !w || !feats || !targets || !grad || n == 0) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DevTmp dw, dx, dt, buf, err, dg, dl;
+    HIPT(dw.alloc(sizeof(float) * MLP_NPARAM), "alloc");
+    HIPT(dx.alloc(sizeof(float) * MLP_IN * n), "alloc");
+    HIPT(dt.alloc(sizeof(float) * n), "alloc");
+    HIPT(buf.alloc(sizeof(float) * mlp_row_floats() * n), "alloc");
+    HIPT(err.alloc(sizeof(float) * n), "alloc");
+    HIPT(dg.alloc(sizeof(float) * MLP_NPARAM), "alloc");
+    HIPT(dl.alloc(sizeof(float)), "alloc");
+    HIPT(hipMemcpy(dw.p, w, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice), "upload");
+    HIPT(hipMemcpy(dx.p, feats, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice), "upload");
+    HIPT(hipMemcpy(dt.p, targets, sizeof(float) * n, hipMemcpyHostToDevice), "upload");
+    HIPT(mlp_grad(dw.as<float>(), dx.as<float>(), dt.as<float>(), nullptr, (i64)n, buf.as<float>(), err.as<float>(),
+                  dg.as<float>(), dl.as<float>(), c->st),
+         "gradient");
+    HIPT(hipMemcpyAsync(grad, dg.p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->st), "download");
+    float l = 0;
+    HIPT(hipMemcpyAsync(&l, dl.p, sizeof(float), hipMemcpyDeviceToHost, c->st), "download");
+    HIPT(hipStreamSynchronize(c->st), "gradient");
+    if (loss) *loss = l;
+    return MDR_OK;
+}
+
+mdr_status mdr_mlp_train(mdr_ctx* c, const float* feats, const float* targets, size_t n, int epochs, int batch,
+                         float lr, unsigned int seed, float* final_loss) {
+    if (!c || !feats || !targets || n == 0 || epochs < 1 || batch < 1 || !(lr > 0)) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    std::vector<float> w0 = c->mlp_host;
+    std::mt19937 rng(seed);
+    if (!c->mlp_ok) {
+        w0.assign(MLP_NPARAM, 0.f);
+        auto fill = [&](int off, int cnt, int fan_in) {
+            std::uniform_real_distribution<float> d(-std::sqrt(6.f / fan_in), std::sqrt(6.f / fan_in));
+            for (int i = 0; i < cnt; ++i) w0[off + i] = d(rng);
+        };
+        fill(MLP_W1, MLP_H * MLP_IN, MLP_IN);
+        fill(MLP_W2, MLP_H * MLP_H, MLP_H);
+        fill(MLP_W3, MLP_H, MLP_H);
+        double mean = 0;
+        for (size_t i = 0; i < n; ++i) mean += targets[i];
+        w0[MLP_B3] = (float)(mean / (double)n);
+    }
+    const size_t bsz = std::min<size_t>((size_t)batch, n);
+    DevTmp dw, dx, dt, buf, err, dg, dm, dv, didx, dl;
+    HIPT(dw.alloc(sizeof(float) * MLP_NPARAM), "alloc");
+    HIPT(dx.alloc(sizeof(float) * MLP_IN * n), "alloc");
+    HIPT(dt.alloc(sizeof(float) * n), "alloc");
+    HIPT(buf.alloc(sizeof(float) * mlp_row_floats() * std::max(bsz, n)), "alloc");
+    HIPT(err.alloc(sizeof(float) * n), "alloc");
+    HIPT(dg.alloc(sizeof(float) * MLP_NPARAM), "alloc");
+    HIPT(dm.alloc(sizeof(float) * MLP_NPARAM), "alloc");
+    HIPT(dv.alloc(sizeof(float) * MLP_NPARAM), "alloc");
+    HIPT(didx.alloc(sizeof(int) * n), "alloc");
+    HIPT(dl.alloc(sizeof(float)), "alloc");
+    HIPT(hipMemcpy(dw.p, w0.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice), "upload");
+    HIPT(hipMemcpy(dx.p, feats, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice), "upload");
+    HIPT(hipMemcpy(dt.p, targets, sizeof(float) * n, hipMemcpyHostToDevice), "upload");
+    HIPT(hipMemset(dm.p, 0, sizeof(float) * MLP_NPARAM), "clear");
+    HIPT(hipMemset(dv.p, 0, sizeof(float) * MLP_NPARAM), "clear");
+    std::vector<int> idx(n);
+    for (size_t i = 0; i < n; ++i) idx[i] = (int)i;
+    int step = 0;
+    for (int e = 0; e < epochs; ++e) {
+        std::shuffle(idx.begin(), idx.end(), rng);
+        HIPT(hipMemcpyAsync(didx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->st), "upload");
+        for (size_t b0 = 0; b0 + bsz <= n; b0 += bsz) {
+            HIPT(mlp_grad(dw.as<float>(), dx.as<float>(), dt.as<float>(), didx.as<int>() + b0, (i64)bsz,
+                          buf.as<float>(), nullptr, dg.as<float>(), nullptr, c->st),
+                 "gradient");
+            HIPT(mlp_adam(dw.as<float>(), dg.as<float>(), dm.as<float>(), dv.as<float>(), lr, ++step, c->st), "adam");
+        }
+        HIPT(hipStreamSynchronize(c->st), "epoch");
+    }
+    // loss over all samples with the final weights
+    HIPT(mlp_grad(dw.as<float>(), dx.as<float>(), dt.as<float>(), nullptr, (i64)n, buf.as<float>(), err.as<float>(),
+                  dg.as<float>(), dl.as<float>(), c->st),
+         "loss");
+    std::vector<float> w(MLP_NPARAM);
+    float l = 0;
+    HIPT(hipMemcpyAsync(w.data(), dw.p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->st), "download");
+    HIPT(hipMemcpyAsync(&l, dl.p, sizeof(float), hipMemcpyDeviceToHost, c->st), "download");
+    HIPT(hipStreamSynchronize(c->st), "train");
+    if (final_loss) *final_loss = l;
+    return set_model(c, w.data(), MLP_NPARAM);
 }
 
 }  // extern "C"

@@ -8,7 +8,7 @@
 namespace mdr {
 
 struct SieveArgs {
-    i64 U, ud, W;
+    i64 U, ud, L;
     i64 seg_first;
     i64 mu_segs;               // segments whose packed mu is written
     const uint8_t* stencil;    // STENCIL_PERIOD + SEG bytes
@@ -19,18 +19,21 @@ struct SieveArgs {
     const uint32_t* sqprimes;  // primes >= 7 whose squares are flagged
     int n_sq, n_sq_small;      // total, and those below 256 (larger ones go with the weights)
     int16_t* Md;
-    u64* cpl;
-    int32_t* cck;
+    CLine* cl;
     uint32_t* mu6;
     i64* segsum;
 };
 
 struct FixupArgs {
-    i64 ud, W;
+    i64 ud, L;
     int16_t* Md;
-    int32_t* cck;
+    CLine* cl;
     const i64* segoff;
 };
+
+// Fixed per-slot cost (cycles) charged by the learned partition for a G
+// evaluation's set-up (frame, segment geometry, constant).
+constexpr long long SLOT_OVERHEAD_CYCLES = 1500;
 
 // Partition of the quotient phase.
 struct PartArgs {
@@ -38,37 +41,50 @@ struct PartArgs {
     int nx;
     i64 total_slots;
     TableParam tp;
-    i64 chunk;                 // static chunk length in positions
-    const float* cost_rate;    // learned: predicted cycles per position per segment class (nullptr = static)
-    double cost_target;        // learned: target cycles per chunk
-    i64* counts;               // per slot chunk count (k_count output)
+    const float* mlp;              // learned cost model (nullptr = static partition)
+    i64* len;                      // per slot positions; position offsets after the scan
+    i64* cost;                     // per slot predicted cycles; cost offsets after the scan
     unsigned long long* type_pos;  // [3]: S1 dense, S1 compressed, S2 positions
-    const i64* offsets;        // per slot first record (k_expand input)
-    i64 nrec;                  // total records
-    ChunkRec* recs;
+    const i64* totals;             // [0] total positions, [1] total predicted cycles
+    i64 P;                         // chunks
+    i64* bounds;                   // [P + 1]
 };
 
 struct QuotArgs {
     const XParam* xp;
+    int nx;
     TableParam tp;
-    const ChunkRec* recs;
-    i64 nrec;
-    unsigned long long* next;  // dynamic work counter
-    u64* out128;               // 2 u64 per X
-    uint32_t* stamps;          // optional per-record cycle counts
-    int dynamic;               // 1: atomic work queue, 0: static round robin
+    const i64* off;                // per slot first global position
+    i64 nslots;
+    const i64* bounds;             // [P + 1]
+    i64 P;
+    unsigned long long* next;      // dynamic work counter
+    u64* out128;                   // 2 u64 per X
+    int dynamic;                   // 1: atomic chunk queue, 0: static round robin
+    StampRec* stamps;              // optional cycle log
+    unsigned long long* nstamps;
+    i64 stamp_cap;
+    int stamp_stride;              // log slots whose hash is 0 mod stride
 };
 
 __global__ void k_sieve(SieveArgs a);
 __global__ void k_table_fixup(FixupArgs a);
-__global__ void k_count(PartArgs a);
-__global__ void k_expand(PartArgs a);
+__global__ void k_slot_len(PartArgs a);
+__global__ void k_cuts(PartArgs a);
 __global__ void k_quot(QuotArgs a);
 
-// Exclusive scan of an i64 array (in place allowed); returns total via *total.
+// Exclusive scan of an i64 array (in place allowed).  The total lands in
+// d_tmp[(n + 2047) / 2048] on the device and, when total != nullptr, on the host.
 hipError_t scan_exclusive_i64(const i64* in, i64* out, i64 n, i64* d_tmp, i64 tmp_len, i64* total,
                               hipStream_t st);
 i64 scan_tmp_len(i64 n);
+
+// Cost-model network (mdr_mlp.hip).
+hipError_t mlp_predict(const float* w, const float* X, i64 n, float* out, hipStream_t st);
+hipError_t mlp_grad(const float* w, const float* X, const float* T, const int* idx, i64 n, float* buf, float* err,
+                    float* grad, float* loss, hipStream_t st);
+hipError_t mlp_adam(float* w, const float* g, float* m, float* v, float lr, int step, hipStream_t st);
+i64 mlp_row_floats();
 
 }  // namespace mdr
 

@@ -1,16 +1,21 @@
-// Quotient phase: partition of every G evaluation into chunks, and the
+// Quotient phase: partition of the G evaluations into chunks and the
 // persistent kernel that evaluates the S1 table sums and the S2 piecewise
-// mod-6 sums of each chunk.
+// mod-6 sums.
 //
 // Reference: the term list and outer reduction (mertens_dr.hpp:1050-1071,
 // :1361-1380, :1469-1485, :1616-1632), eval_single_S / eval_comb2
 // (:1081-1354, :1490-1614), sum_dense_quotients (:88-128),
 // reconstruct_compressed (:741-752) and run_s2_fast (:484-617).  The
-// reference gives one worker per outer term; here every S1 run and S2 piece
-// of every term is cut into bounded chunks fed to persistent waves.
+// reference gives one worker per outer term.  Here every G evaluation of every
+// X is laid out in one global position space (position 0 of a G evaluation
+// is its constant, then its S1 runs and S2 pieces in order); a boundary array
+// cuts that space into P chunks -- at equal positions (static partition) or
+// at equal predicted cycles (learned partition) -- and persistent waves sum
+// the chunks.
 #include <hip/hip_runtime.h>
 #include "mdr_device.hpp"
 #include "mdr_kernels.hpp"
+#include "mdr_mlp.hpp"
 
 namespace mdr {
 
@@ -26,22 +31,22 @@ __device__ __forceinline__ int find_x(const XParam* xp, int nx, i64 G) {
     return lo;
 }
 
+// last index i in [0, n) with a[i] <= v (a non-decreasing, a[0] <= v)
+__device__ __forceinline__ i64 last_le(const i64* a, i64 n, i64 v) {
+    i64 lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const i64 mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 }  // namespace
 
-// Total positions of one G evaluation (its segments concatenated in order).
-__device__ __forceinline__ i64 frame_positions(const Frame& f, const XParam& xp, const TableParam& tp) {
-    i64 L = 0;
-    const int ns = frame_nseg(f);
-    for (int i = 0; i < ns; ++i) L += frame_seg(f, xp, tp, i).count;
-    return L;
-}
-
-__device__ __forceinline__ i64 frame_chunks(const PartArgs& a, i64 L) {
-    const i64 c = (L + a.chunk - 1) / a.chunk;
-    return c > 0 ? c : 1;   // the constant still needs a record
-}
-
-__global__ void k_count(PartArgs a) {
+// Per-slot length (1 + positions of its segments; 0 for an empty slot) and,
+// for the learned partition, predicted cycles (rounded per cost piece).
+__global__ void k_slot_len(PartArgs a) {
     const i64 G = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 pos[3] = {0, 0, 0};
     if (G < a.total_slots) {
@@ -49,15 +54,25 @@ __global__ void k_count(PartArgs a) {
         const XParam xp = a.xp[x];
         const Frame f = make_frame(xp, a.tp.mu6, G - xp.slot_base);
         const int ns = frame_nseg(f);
-        i64 L = 0;
-        for (int i = 0; i < ns; ++i) {
-            const Seg s = frame_seg(f, xp, a.tp, i);
-            L += s.count;
-            pos[s.type == SEG_S1_DENSE ? 0 : s.type == SEG_S1_COMP ? 1 : 2] += (u64)s.count;
+        i64 L = 0, C = 0;
+        if (ns > 0) {
+            L = 1;
+            C = SLOT_OVERHEAD_CYCLES;
+            for (int i = 0; i < ns; ++i) {
+                const Seg sg = frame_seg(f, xp, a.tp, i);
+                L += sg.count;
+                pos[sg.type == SEG_S1_DENSE ? 0 : sg.type == SEG_S1_COMP ? 1 : 2] += (u64)sg.count;
+                if (a.mlp)
+                    for (i64 p = 0; p < sg.count;) {
+                        const i64 pe = piece_end(sg, p);
+                        C += (i64)__builtin_rint(piece_cost(a.mlp, f.y, sg, p, pe));
+                        p = pe;
+                    }
+            }
         }
-        a.counts[G] = ns > 0 ? frame_chunks(a, L) : 0;
+        a.len[G] = L;
+        if (a.mlp) a.cost[G] = C;
     }
-    // per-type position totals: one atomic per wave and type
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         u64 v = pos[k];
@@ -66,36 +81,49 @@ __global__ void k_count(PartArgs a) {
     }
 }
 
-// One thread per record: the record's slot is the last slot whose first
-// record is <= r (slots with no records share their successor's offset).
-// Chunk j of c covers positions [j*L/c, (j+1)*L/c) of the concatenated
-// segments; the first chunk also adds the G evaluation's constant.
-__global__ void k_expand(PartArgs a) {
-    const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.nrec) return;
-    i64 lo = 0, hi = a.total_slots - 1;
-    while (lo < hi) {
-        const i64 mid = (lo + hi + 1) >> 1;
-        if (a.offsets[mid] <= r) lo = mid;
-        else hi = mid - 1;
+// Chunk boundaries B[0..P]: static c*T/P, or the position where the running
+// predicted cost reaches c*C/P (monotone in c, B[0] = 0, B[P] = T).
+__global__ void k_cuts(PartArgs a) {
+    const i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > a.P) return;
+    const i64 T = a.totals[0];
+    if (!a.mlp || c == 0 || c == a.P) {
+        a.bounds[c] = (i64)((u128)(u64)c * (u64)T / (u64)a.P);
+        return;
     }
-    const i64 G = lo;
-    const i64 o = a.offsets[G];
-    const i64 c = (G + 1 < a.total_slots ? a.offsets[G + 1] : a.nrec) - o;
-    const i64 j = r - o;
-    const int x = find_x(a.xp, a.nx, G);
-    const XParam xp = a.xp[x];
-    const i64 g = G - xp.slot_base;
-    const Frame f = make_frame(xp, a.tp.mu6, g);
-    const i64 L = frame_positions(f, xp, a.tp);
-    ChunkRec rec;
-    rec.x = (uint32_t)x;
-    rec.g = (uint32_t)g;
-    rec.seg = j == 0 ? 0x100u : 0u;
-    rec.pad = 0;
-    rec.p_lo = (u64)((u128)j * (u64)L / (u64)c);
-    rec.p_hi = (u64)((u128)(j + 1) * (u64)L / (u64)c);
-    a.recs[r] = rec;
+    const i64 Ctot = a.totals[1];
+    const i64 target = (i64)((u128)(u64)c * (u64)Ctot / (u64)a.P);
+    const i64 G = last_le(a.cost, a.total_slots, target);
+    const i64 base = a.len[G];
+    const i64 Lg = (G + 1 < a.total_slots ? a.len[G + 1] : T) - base;
+    i64 local = target - a.cost[G];
+    i64 pos = 0;
+    if (Lg > 0 && local >= SLOT_OVERHEAD_CYCLES) {
+        local -= SLOT_OVERHEAD_CYCLES;
+        pos = 1;
+        const int x = find_x(a.xp, a.nx, G);
+        const XParam xp = a.xp[x];
+        const Frame f = make_frame(xp, a.tp.mu6, G - xp.slot_base);
+        const int ns = frame_nseg(f);
+        bool done = false;
+        for (int i = 0; i < ns && !done; ++i) {
+            const Seg sg = frame_seg(f, xp, a.tp, i);
+            for (i64 p = 0; p < sg.count;) {
+                const i64 pe = piece_end(sg, p);
+                const i64 pc = (i64)__builtin_rint(piece_cost(a.mlp, f.y, sg, p, pe));
+                if (local < pc) {
+                    pos += (i64)((u128)(u64)(pe - p) * (u64)local / (u64)(pc > 0 ? pc : 1));
+                    done = true;
+                    break;
+                }
+                local -= pc;
+                pos += pe - p;
+                p = pe;
+            }
+        }
+        if (pos > Lg) pos = Lg;
+    }
+    a.bounds[c] = base + pos;
 }
 
 // ---------------------------------------------------------------------------
@@ -132,11 +160,7 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
     for (u64 i = lane; i < cnt; i += 64) {
         const u64 n = nlo + (u64)ST * i;
         const u64 q = qdiv_mid(y, dy, n, rcp_refined((double)n));
-        const u64 off = q - (u64)tp.ud - 1;
-        const u64 w = off >> 6;
-        const u64 nz = tp.cpl[2 * w], ng = tp.cpl[2 * w + 1];
-        const u64 mask = ~0ull >> (63 - (off & 63));
-        s += tp.cck[w] + (int)__popcll(nz & mask) - 2 * (int)__popcll(ng & mask);
+        s += comp_M(tp.cl, q - (u64)tp.ud - 1);
     }
     return s;
 }
@@ -211,58 +235,89 @@ __device__ __forceinline__ i64 fetch_next(unsigned long long* ctr, int lane) {
     if (lane == 0) v = atomicAdd(ctr, 1ull);
     return uniform64((i64)__shfl(v, 0, 64));
 }
+__device__ __forceinline__ void flush(u64* out128, i64 x, u64& lo, i64& hi, int lane) {
+    wave_reduce128(lo, hi);
+    if (lane == 0 && (lo | (u64)hi)) atomic_add128(out128 + 2 * x, lo, hi);
+    lo = 0;
+    hi = 0;
+}
 }  // namespace
 
 __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
     const int lane = threadIdx.x & 63;
     const i64 wave = ((i64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const i64 nwaves = ((i64)gridDim.x * blockDim.x) >> 6;
+    const i64 T = a.bounds[a.P];
     u64 acc_lo = 0;
     i64 acc_hi = 0;
-    i64 cur_x = -1;
-    i64 r = a.dynamic ? fetch_next(a.next, lane) : uniform64(wave);
-    while (r < a.nrec) {
-        const ChunkRec rec = a.recs[r];
-        if ((i64)rec.x != cur_x) {
-            if (cur_x >= 0) {
-                wave_reduce128(acc_lo, acc_hi);
-                if (lane == 0) atomic_add128(a.out128 + 2 * cur_x, acc_lo, acc_hi);
-                acc_lo = 0;
-                acc_hi = 0;
+    i64 cur_x = -1, x_end = 0;
+    XParam xp{};
+    for (i64 c = a.dynamic ? fetch_next(a.next, lane) : uniform64(wave); c < a.P;
+         c = a.dynamic ? fetch_next(a.next, lane) : c + nwaves) {
+        const i64 b0 = a.bounds[c], b1 = a.bounds[c + 1];
+        if (b0 >= b1) continue;
+        for (i64 G = last_le(a.off, a.nslots, b0); G < a.nslots; ++G) {
+            const i64 og = a.off[G];
+            if (og >= b1) break;
+            const i64 og1 = G + 1 < a.nslots ? a.off[G + 1] : T;
+            if (og1 <= og) continue;
+            if (G >= x_end || cur_x < 0) {
+                if (cur_x >= 0) flush(a.out128, cur_x, acc_lo, acc_hi, lane);
+                cur_x = find_x(a.xp, a.nx, G);
+                xp = a.xp[cur_x];
+                x_end = cur_x + 1 < a.nx ? a.xp[cur_x + 1].slot_base : a.nslots;
             }
-            cur_x = rec.x;
-        }
-        const long long t0 = a.stamps ? clock64() : 0;
-        const XParam xp = a.xp[rec.x];
-        const Frame f = make_frame(xp, a.tp.mu6, rec.g);
-        const int ns = frame_nseg(f);
-        i64 s0 = 0;
-        for (int i = 0; i < ns && s0 < (i64)rec.p_hi; ++i) {
-            const Seg sg = frame_seg(f, xp, a.tp, i);
-            const i64 s1 = s0 + sg.count;
-            const i64 lo = (i64)rec.p_lo > s0 ? (i64)rec.p_lo : s0;
-            const i64 hi = (i64)rec.p_hi < s1 ? (i64)rec.p_hi : s1;
-            if (lo < hi) {
-                const i64 part = run_seg(a.tp, sg, (u64)f.y, (u64)(lo - s0), (u64)(hi - s0), lane);
-                const i64 v = (f.coef * sg.sign) * part;
-                add128(acc_lo, acc_hi, (u64)v, v >> 63);
+            const Frame f = make_frame(xp, a.tp.mu6, G - xp.slot_base);
+            const i64 lo = (b0 > og ? b0 : og) - og, hi = (b1 < og1 ? b1 : og1) - og;
+            if (lo == 0 && lane == 0) {
+                const i128 cst = (i128)f.coef * frame_const(f, a.tp);
+                add128(acc_lo, acc_hi, (u64)cst, (i64)(cst >> 64));
             }
-            s0 = s1;
+            const bool stamp = a.stamps && (((u64)G * 0x9E3779B97F4A7C15ull) >> 40) % (u64)a.stamp_stride == 0;
+            const int ns = frame_nseg(f);
+            i64 s0 = 1;
+            for (int i = 0; i < ns && s0 < hi; ++i) {
+                const Seg sg = frame_seg(f, xp, a.tp, i);
+                const i64 s1 = s0 + sg.count;
+                const i64 l = lo > s0 ? lo : s0, h = hi < s1 ? hi : s1;
+                if (l < h) {
+                    const int sgn = f.coef * sg.sign;
+                    if (!stamp) {
+                        const i64 v = sgn * run_seg(a.tp, sg, (u64)f.y, (u64)(l - s0), (u64)(h - s0), lane);
+                        add128(acc_lo, acc_hi, (u64)v, v >> 63);
+                    } else {
+                        // time each cost piece separately (training data)
+                        for (i64 p = 0; p < sg.count;) {
+                            const i64 pe = piece_end(sg, p);
+                            const i64 pl = (l - s0) > p ? (l - s0) : p, ph = (h - s0) < pe ? (h - s0) : pe;
+                            if (pl < ph) {
+                                const long long t0 = clock64();
+                                const i64 v = sgn * run_seg(a.tp, sg, (u64)f.y, (u64)pl, (u64)ph, lane);
+                                add128(acc_lo, acc_hi, (u64)v, v >> 63);
+                                const long long dt = clock64() - t0;
+                                if (lane == 0) {
+                                    const unsigned long long k = atomicAdd(a.nstamps, 1ull);
+                                    if ((i64)k < a.stamp_cap) {
+                                        StampRec r;
+                                        r.y = f.y;
+                                        r.start = seg_value(sg, pl);
+                                        r.end = seg_value(sg, ph - 1);
+                                        r.kind = (uint32_t)seg_kind(sg);
+                                        r.cycles = (uint32_t)(dt > 0xffffffffll ? 0xffffffffll : (dt < 0 ? 0 : dt));
+                                        r.count = (u64)(ph - pl);
+                                        a.stamps[k] = r;
+                                    }
+                                }
+                            }
+                            p = pe;
+                        }
+                    }
+                }
+                s0 = s1;
+            }
         }
-        if ((rec.seg & 0x100u) && lane == 0) {
-            const i128 c = (i128)f.coef * frame_const(f, a.tp);
-            add128(acc_lo, acc_hi, (u64)c, (i64)(c >> 64));
-        }
-        if (a.stamps && lane == 0) {
-            const long long dt = clock64() - t0;
-            a.stamps[r] = (uint32_t)(dt > 0xffffffffll ? 0xffffffffll : dt);
-        }
-        r = a.dynamic ? fetch_next(a.next, lane) : r + nwaves;
     }
-    if (cur_x >= 0) {
-        wave_reduce128(acc_lo, acc_hi);
-        if (lane == 0) atomic_add128(a.out128 + 2 * cur_x, acc_lo, acc_hi);
-    }
+    if (cur_x >= 0) flush(a.out128, cur_x, acc_lo, acc_hi, lane);
 }
 
 // ---------------------------------------------------------------------------

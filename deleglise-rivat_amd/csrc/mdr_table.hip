@@ -39,11 +39,13 @@ __device__ __forceinline__ int8_t finalize(uint8_t a, u64 n) {
 
 __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     __shared__ uint32_t acc32[SEG / 4];
+    __shared__ uint32_t nib32[SEG / 16];   // per dword: nonzero nibble | negative nibble << 4
     __shared__ int32_t gsum[GROUPS];
     __shared__ int32_t gpre[GROUPS];
     __shared__ uint32_t small_off[64];
     __shared__ uint32_t sq_off[64];
     uint8_t* acc = reinterpret_cast<uint8_t*>(acc32);
+    uint8_t* nib = reinterpret_cast<uint8_t*>(nib32);
 
     const i64 s = (i64)blockIdx.x + a.seg_first;
     const int tid = threadIdx.x;
@@ -131,39 +133,66 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
     __syncthreads();
 
-    // 5. finalise bytes to mu in {-1, 0, 1}
+    // 5. finalise to mu in {-1, 0, 1}, four values per dword (SWAR): a flag
+    //    gives 0; otherwise mu = +1 exactly when the parity bit differs from
+    //    [acc > floor(log2 n)].  Dwords at n = 1, 2, past U or across a power
+    //    of two take the per-byte path.  Each dword also leaves a byte of its
+    //    nonzero and negative bits (low/high nibble) for the plane builder.
     for (int i = tid; i < SEG / 4; i += SIEVE_THREADS) {
-        uint32_t v = acc32[i];
-        uint32_t out = 0;
+        const uint32_t x = acc32[i];
+        const u64 n0 = first + 4 * (u64)i;
+        const unsigned t0 = 63u - (unsigned)__clzll(n0);
+        uint32_t mu;
+        if (n0 > 2 && n0 + 3 <= (u64)a.U && t0 == 63u - (unsigned)__clzll(n0 + 3)) {
+            const uint32_t gt = ((x & 0x7f7f7f7fu) + (127u - t0) * 0x01010101u) & 0x80808080u;
+            const uint32_t t = (x & 0x01010101u) ^ (gt >> 7);
+            mu = (0xffffffffu ^ (t * 0xfeu)) & ~(((x >> 7) & 0x01010101u) * 0xffu);
+        } else {
+            mu = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const u64 n = first + 4 * (u64)i + b;
-            int8_t m = 0;
-            if (n <= (u64)a.U) {
-                m = finalize((uint8_t)(v >> (8 * b)), n);
-                if (n == 1) m = 1;
-                if (n == 2) m = -1;
+            for (int b = 0; b < 4; ++b) {
+                const u64 n = n0 + b;
+                int8_t m = 0;
+                if (n <= (u64)a.U) {
+                    m = finalize((uint8_t)(x >> (8 * b)), n);
+                    if (n == 1) m = 1;
+                    if (n == 2) m = -1;
+                }
+                mu |= (uint32_t)(uint8_t)m << (8 * b);
             }
-            out |= (uint32_t)(uint8_t)m << (8 * b);
         }
-        acc32[i] = out;
+        acc32[i] = mu;
+        // bits at 0, 8, 16, 24 gathered to bits 0..3 (the four partial
+        // products of the multiply land on distinct bits, so nothing carries)
+        const uint32_t nzn = (((mu & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
+        const uint32_t ngn = ((((mu >> 7) & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
+        nib[i] = (uint8_t)(nzn | (ngn << 4));
     }
     __syncthreads();
 
-    // 6. group sums and compressed planes (ballots over 64 values)
-    for (int g = wave; g < GROUPS; g += SIEVE_THREADS / 64) {
-        const int8_t v = (int8_t)acc[g * 64 + lane];
-        const u64 nz = __ballot(v != 0);
-        const u64 ng = __ballot(v < 0);
-        if (lane == 0) gsum[g] = (int)__popcll(nz) - 2 * (int)__popcll(ng);
-        const u64 n0 = first + (u64)g * 64;
-        if (n0 > (u64)a.ud && n0 <= (u64)a.U && lane < 2) {
-            const i64 w = (i64)(n0 - (u64)a.ud - 1) >> 6;
-            a.cpl[2 * w + lane] = lane ? ng : nz;
+    // 6. one lane per 64-value word: bit planes from 16 nibble bytes, word sums
+    for (int k = tid; k < GROUPS; k += SIEVE_THREADS) {
+        const uint4 b = reinterpret_cast<const uint4*>(nib32)[k];
+        auto lo16 = [](uint32_t v) {
+            v &= 0x0f0f0f0fu;
+            v = (v | (v >> 4)) & 0x00ff00ffu;
+            return (v | (v >> 8)) & 0xffffu;
+        };
+        const u64 nz = (u64)lo16(b.x) | ((u64)lo16(b.y) << 16) | ((u64)lo16(b.z) << 32) | ((u64)lo16(b.w) << 48);
+        const u64 ng = (u64)lo16(b.x >> 4) | ((u64)lo16(b.y >> 4) << 16) | ((u64)lo16(b.z >> 4) << 32) |
+                       ((u64)lo16(b.w >> 4) << 48);
+        gsum[k] = (int)__popcll(nz) - 2 * (int)__popcll(ng);
+        const u64 n0 = first + (u64)k * 64;
+        if (n0 > (u64)a.ud && n0 <= (u64)a.U) {
+            const uint32_t w = (uint32_t)((n0 - (u64)a.ud - 1) >> 6);
+            const uint32_t line = w / 3u, sub = w - 3u * line;
+            u64* pl = a.cl[line].pl + 2 * sub;
+            pl[0] = nz;
+            pl[1] = ng;
         }
     }
     __syncthreads();
-    // exclusive scan of the 768 group sums by wave 0 (12 per lane)
+    // exclusive scan of the 768 word sums by wave 0 (12 per lane)
     if (wave == 0) {
         int32_t loc[GROUPS / 64];
         int32_t t = 0;
@@ -182,22 +211,36 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
     __syncthreads();
 
-    // 7. dense prefix values and compressed checkpoints (segment-local)
-    for (int g = wave; g < GROUPS; g += SIEVE_THREADS / 64) {
-        const u64 n0 = first + (u64)g * 64;
-        if (n0 > (u64)a.U) break;
-        if (n0 <= (u64)a.ud) {
-            int32_t v = (int8_t)acc[g * 64 + lane];
+    // 7. checkpoints of compressed words (segment-local; the fix-up adds the
+    //    segment offset) and dense prefix values, one lane per dword
+    for (int k = tid; k < GROUPS; k += SIEVE_THREADS) {
+        const u64 n0 = first + (u64)k * 64;
+        if (n0 > (u64)a.ud && n0 <= (u64)a.U) {
+            const uint32_t w = (uint32_t)((n0 - (u64)a.ud - 1) >> 6);
+            const uint32_t line = w / 3u, sub = w - 3u * line;
+            a.cl[line].ck[sub] = gpre[k];
+        }
+    }
+    if (first <= (u64)a.ud) {
+        for (int i = tid; i < SEG / 4; i += SIEVE_THREADS) {
+            const uint32_t x = acc32[i];
+            const int b0 = (int8_t)x, b1 = (int8_t)(x >> 8), b2 = (int8_t)(x >> 16), b3 = (int8_t)(x >> 24);
+            const int s4 = b0 + b1 + b2 + b3;
+            // exclusive prefix over the word's 16 dwords (aligned groups of 16 lanes)
+            int inc = s4;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int32_t o = __shfl_up(v, d, 64);
-                if (lane >= d) v += o;
+            for (int d = 1; d < 16; d <<= 1) {
+                const int o = __shfl_up(inc, d, 16);
+                if ((lane & 15) >= d) inc += o;
             }
-            const u64 n = n0 + lane;
-            if (n <= (u64)a.ud) a.Md[n] = (int16_t)(gpre[g] + v);
-        } else if (lane == 0) {
-            const i64 w = (i64)(n0 - (u64)a.ud - 1) >> 6;
-            a.cck[w] = gpre[g];
+            int v = gpre[i >> 4] + inc - s4;
+            const u64 n = first + 4 * (u64)i;
+            const int bb[4] = {b0, b1, b2, b3};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                v += bb[t];
+                if (n + t <= (u64)a.ud) a.Md[n + t] = (int16_t)v;
+            }
         }
     }
 
@@ -226,9 +269,12 @@ __global__ void k_table_fixup(FixupArgs a) {
         const i64 s = (n - 1) / SEG;
         a.Md[n] = (int16_t)(a.Md[n] + (int32_t)a.segoff[s]);
     }
-    if (i < a.W) {
-        const i64 s = (a.ud + 64 * i) / SEG;
-        a.cck[i] = a.cck[i] + (int32_t)a.segoff[s];
+    if (i < a.L) {
+        const int32_t add = (int32_t)a.segoff[(a.ud + (i64)LINE_VALUES * i) / SEG];
+        int32_t* ck = a.cl[i].ck;
+        ck[0] += add;
+        ck[1] += add;
+        ck[2] += add;
     }
 }
 

@@ -2,10 +2,12 @@
 //
 // The cache is a text file written by the project's autotuner
 // (deleglise-rivat_amd/autotune.py): one line per decade of X,
-//     decade  fx  cx  chunk
+//     decade  fx  cx  chunks  u_dense
 // where u = fx * (X / ln ln X)^(2/3) (the reference's cutoff form,
-// mertens_dr.hpp:1401-1419, with a GPU-specific factor).  The file named by
-// MDR_TUNE_FILE is read once; without it the built-in defaults apply.
+// mertens_dr.hpp:1401-1419, with a GPU-specific factor), chunks is the
+// quotient-phase chunk count (0 = automatic) and u_dense the dense-tier
+// limit.  The file named by MDR_TUNE_FILE, else tune_cache.txt beside the
+// library, is read once; without it the built-in defaults apply.
 #include "mdr_tune.hpp"
 
 #include <algorithm>
@@ -13,7 +15,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <string>
 #include <vector>
+
+#include <dlfcn.h>
 
 namespace mdr {
 
@@ -23,27 +28,36 @@ struct Entry {
     int decade;
     double fx, cx;
     i64 chunk;
+    i64 ud;
 };
 
 std::once_flag g_once;
 std::vector<Entry> g_entries;
-i64 g_default_chunk = 32768;
 
 void load() {
-    const char* path = std::getenv("MDR_TUNE_FILE");
-    if (!path || !*path) return;
-    FILE* f = std::fopen(path, "r");
+    std::string path;
+    if (const char* e = std::getenv("MDR_TUNE_FILE")) path = e;
+    else {
+        Dl_info info;
+        if (dladdr((void*)&tune_chunk, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tune_cache.txt";
+        }
+    }
+    if (path.empty()) return;
+    FILE* f = std::fopen(path.c_str(), "r");
     if (!f) return;
     char line[256];
     while (std::fgets(line, sizeof line, f)) {
         Entry e{};
-        long long ch = 0;
+        long long ch = 0, ud = 0;
         if (line[0] == '#') continue;
-        if (std::sscanf(line, "%d %lf %lf %lld", &e.decade, &e.fx, &e.cx, &ch) == 4) {
-            if (e.fx > 0.05 && e.fx < 100 && e.cx > 0.1 && e.cx < 5.0 && ch >= 1024) {
-                e.chunk = ch;
-                g_entries.push_back(e);
-            }
+        const int k = std::sscanf(line, "%d %lf %lf %lld %lld", &e.decade, &e.fx, &e.cx, &ch, &ud);
+        if (k >= 4 && e.fx > 0.05 && e.fx < 100 && e.cx > 0.1 && e.cx < 5.0 && ch >= 0) {
+            e.chunk = ch;
+            e.ud = k == 5 && ud >= 0 ? ud : 0;
+            g_entries.push_back(e);
         }
     }
     std::fclose(f);
@@ -58,31 +72,47 @@ i64 isqrt_t(i64 n) {
 
 }  // namespace
 
+// Built-in defaults, measured on MI355X at X = 1e17 (sweep of u, the dense
+// tier and cx): u = 0.7 (X / ln ln X)^(2/3) (6.4e10 at 1e17), a dense int16
+// tier of about 2^20 entries (L2-resident; everything above it in 64-byte
+// compressed lines, which keeps the hot part of the table in the last-level
+// cache) and cx = 2.0, which moves work from table gathers to S2.
 TuneChoice tune_lookup(i64 X, bool streaming) {
+    (void)streaming;
     std::call_once(g_once, load);
     const int dec = (int)std::floor(std::log10((double)X));
-    double fx = streaming ? 4.0 : 1.0;
-    double cx = X >= 1000000000000000LL ? 1.35 : X >= 10000000000000LL ? 1.15 : 0.95;
+    double fx = 0.7;
+    double cx = X >= 10000000000000LL ? 2.0 : X >= 100000000000LL ? 0.95 : 0.70;
+    i64 ud = 1048512;
     for (const Entry& e : g_entries)
-        if (e.decade == dec) { fx = e.fx; cx = e.cx; }
+        if (e.decade == dec) {
+            fx = e.fx;
+            cx = e.cx;
+            if (e.ud > 0) ud = e.ud;
+        }
     const double ll = std::log(std::max(2.0, std::log((double)X)));
     i64 u = (i64)(fx * std::pow((double)X / ll, 2.0 / 3.0));
     const i64 S = isqrt_t(X);
     u = std::max(u, 3 * S);
     const i64 ends = (i64)(cx * std::sqrt((double)X)) + 1000;
     u = std::max(u, ends + 64);
-    u = std::min<i64>(u, 64000000000LL);
-    u = ((u + 255) / 256) * 256;
+    u = std::min<i64>(u, 200000000000LL);
+    u = ((u + 191) / 192) * 192;
     TuneChoice t;
     t.u = u;
     t.cx = cx;
-    t.ud = std::min(u, ((std::max<i64>(ends, 128000000LL) + 255) / 256) * 256);
+    t.ud = std::min(u, (ud / 192) * 192);
     return t;
 }
 
-i64 tune_chunk() {
+i64 tune_chunks(i64 X) {
     std::call_once(g_once, load);
-    return g_default_chunk;
+    const int dec = (int)std::floor(std::log10((double)(X > 1 ? X : 1)));
+    for (const Entry& e : g_entries)
+        if (e.decade == dec) return e.chunk;
+    return 0;
 }
+
+i64 tune_chunk() { return tune_chunks(1); }
 
 }  // namespace mdr

@@ -23,7 +23,12 @@ STATUS = {0: "MDR_OK", 1: "MDR_ERR_INVALID", 2: "MDR_ERR_HIP", 3: "MDR_ERR_NOMEM
 # Every symbol include/mertens_hip.h declares.
 EXPORTS = ["mdr_default_opts", "mdr_init", "mdr_shutdown", "mdr_last_error", "mdr_mertens",
            "mdr_mertens_batch", "mdr_plan", "mdr_table_build", "mdr_table_read_M",
-           "mdr_table_read_mu", "mdr_s2_interval", "mdr_s1_sum", "mdr_g_eval"]
+           "mdr_table_read_mu", "mdr_s2_interval", "mdr_s1_sum", "mdr_g_eval",
+           "mdr_cost_model_set", "mdr_cost_model_get", "mdr_cost_model_load", "mdr_cost_model_save",
+           "mdr_cycle_log", "mdr_mlp_features", "mdr_mlp_predict", "mdr_mlp_grad", "mdr_mlp_train"]
+
+MLP_NPARAM = 1281
+MLP_NFEAT = 5
 
 
 class Opts(ctypes.Structure):
@@ -31,7 +36,8 @@ class Opts(ctypes.Structure):
                 ("fx", ctypes.c_double), ("u_cap_dense", ctypes.c_longlong), ("fx_set", ctypes.c_int),
                 ("u_cap_dense_set", ctypes.c_int), ("partition", ctypes.c_int), ("params", ctypes.c_int),
                 ("sched", ctypes.c_int), ("host_threads", ctypes.c_int), ("chunk", ctypes.c_longlong),
-                ("record_cycles", ctypes.c_int), ("no_table_reuse", ctypes.c_int)]
+                ("record_cycles", ctypes.c_int), ("no_table_reuse", ctypes.c_int), ("tune_u", ctypes.c_longlong),
+                ("tune_ud", ctypes.c_longlong), ("tune_cx", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -43,10 +49,16 @@ class Stats(ctypes.Structure):
                 ("mu_limit", ctypes.c_longlong), ("W", ctypes.c_longlong), ("u", ctypes.c_longlong),
                 ("N", ctypes.c_longlong), ("slots", ctypes.c_longlong), ("chunks", ctypes.c_longlong),
                 ("cx", ctypes.c_double), ("s1_terms", ctypes.c_longlong), ("s2_terms", ctypes.c_longlong),
-                ("table_bytes", ctypes.c_double), ("s1_comp_terms", ctypes.c_longlong)]
+                ("table_bytes", ctypes.c_double), ("s1_comp_terms", ctypes.c_longlong),
+                ("partition_used", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Stamp(ctypes.Structure):
+    _fields_ = [("y", ctypes.c_longlong), ("start", ctypes.c_longlong), ("end", ctypes.c_longlong),
+                ("kind", ctypes.c_uint), ("cycles", ctypes.c_uint), ("count", ctypes.c_ulonglong)]
 
 
 class MdrError(RuntimeError):
@@ -93,6 +105,18 @@ def load():
     lib.mdr_s1_sum.restype = i32
     lib.mdr_g_eval.argtypes = [vp, ll, i32, ll, dbl, P(ll), P(ll)]
     lib.mdr_g_eval.restype = i32
+    fp = P(ctypes.c_float)
+    sz = ctypes.c_size_t
+    lib.mdr_cost_model_set.argtypes, lib.mdr_cost_model_set.restype = [vp, fp, i32], i32
+    lib.mdr_cost_model_get.argtypes, lib.mdr_cost_model_get.restype = [vp, fp, i32], i32
+    lib.mdr_cost_model_load.argtypes, lib.mdr_cost_model_load.restype = [vp, ctypes.c_char_p], i32
+    lib.mdr_cost_model_save.argtypes, lib.mdr_cost_model_save.restype = [vp, ctypes.c_char_p], i32
+    lib.mdr_cycle_log.argtypes, lib.mdr_cycle_log.restype = [vp, P(Stamp), sz, P(sz)], i32
+    lib.mdr_mlp_features.argtypes, lib.mdr_mlp_features.restype = [P(Stamp), sz, fp, fp], i32
+    lib.mdr_mlp_predict.argtypes, lib.mdr_mlp_predict.restype = [vp, fp, fp, sz, fp], i32
+    lib.mdr_mlp_grad.argtypes, lib.mdr_mlp_grad.restype = [vp, fp, fp, fp, sz, fp, fp], i32
+    lib.mdr_mlp_train.argtypes = [vp, fp, fp, sz, i32, i32, ctypes.c_float, ctypes.c_uint, fp]
+    lib.mdr_mlp_train.restype = i32
     _lib = lib
     return lib
 
@@ -103,12 +127,14 @@ def _i128(lo, hi):
 
 def make_opts(strategy=STRATEGY_AUTO, cx=0.0, u_cap=0, fx=0.0, u_cap_dense=0, fx_set=0,
               u_cap_dense_set=0, partition=PARTITION_STATIC, params=PARAMS_REFERENCE,
-              sched=SCHED_DYNAMIC, chunk=0, record_cycles=0, no_table_reuse=0):
+              sched=SCHED_DYNAMIC, chunk=0, record_cycles=0, no_table_reuse=0, tune_u=0, tune_ud=0,
+              tune_cx=0.0):
     o = Opts()
     load().mdr_default_opts(ctypes.byref(o))
     o.strategy, o.cx, o.u_cap, o.fx, o.u_cap_dense = strategy, cx, u_cap, fx, u_cap_dense
     o.fx_set, o.u_cap_dense_set, o.partition, o.params = fx_set, u_cap_dense_set, partition, params
     o.sched, o.chunk, o.record_cycles, o.no_table_reuse = sched, chunk, record_cycles, no_table_reuse
+    o.tune_u, o.tune_ud, o.tune_cx = tune_u, tune_ud, tune_cx
     return o
 
 
@@ -244,3 +270,84 @@ class Engine:
         self._check(self.lib.mdr_g_eval(self.ctx, y, 1 if comb2 else 0, u, cx, ctypes.byref(lo), ctypes.byref(hi)),
                     "g_eval")
         return _i128(lo, hi)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def mlp_features(stamps):
+    """Features (n x 5) and targets log2(cycles/count) of cycle-log records."""
+    import numpy as np
+    n = len(stamps)
+    f = np.empty((n, MLP_NFEAT), dtype=np.float32)
+    t = np.empty(n, dtype=np.float32)
+    arr = (Stamp * n)(*stamps) if not isinstance(stamps, ctypes.Array) else stamps
+    st = load().mdr_mlp_features(arr, n, _fptr(f), _fptr(t))
+    if st != 0:
+        raise MdrError(f"mdr_mlp_features: {STATUS.get(st, st)}")
+    return f, t
+
+
+class _CostModel:
+    """Cost-model methods of Engine."""
+
+    def cost_model_set(self, w):
+        import numpy as np
+        if w is None:
+            self._check(self.lib.mdr_cost_model_set(self.ctx, None, 0), "cost_model_set")
+            return
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.lib.mdr_cost_model_set(self.ctx, _fptr(w), len(w)), "cost_model_set")
+
+    def cost_model_get(self):
+        import numpy as np
+        w = np.empty(MLP_NPARAM, dtype=np.float32)
+        self._check(self.lib.mdr_cost_model_get(self.ctx, _fptr(w), MLP_NPARAM), "cost_model_get")
+        return w
+
+    def cost_model_load(self, path):
+        self._check(self.lib.mdr_cost_model_load(self.ctx, path.encode()), "cost_model_load")
+
+    def cost_model_save(self, path):
+        self._check(self.lib.mdr_cost_model_save(self.ctx, path.encode()), "cost_model_save")
+
+    def cycle_log(self):
+        n = ctypes.c_size_t()
+        self._check(self.lib.mdr_cycle_log(self.ctx, None, 0, ctypes.byref(n)), "cycle_log")
+        arr = (Stamp * max(1, n.value))()
+        self._check(self.lib.mdr_cycle_log(self.ctx, arr, n.value, ctypes.byref(n)), "cycle_log")
+        return arr[:n.value]
+
+    def mlp_predict(self, w, feats):
+        import numpy as np
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        out = np.empty(len(feats), dtype=np.float32)
+        self._check(self.lib.mdr_mlp_predict(self.ctx, _fptr(w), _fptr(feats), len(feats), _fptr(out)), "mlp_predict")
+        return out
+
+    def mlp_grad(self, w, feats, targets):
+        import numpy as np
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        targets = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.empty(MLP_NPARAM, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self.lib.mdr_mlp_grad(self.ctx, _fptr(w), _fptr(feats), _fptr(targets), len(feats), _fptr(g),
+                                          ctypes.byref(loss)), "mlp_grad")
+        return g, loss.value
+
+    def mlp_train(self, feats, targets, epochs=30, batch=1024, lr=1e-3, seed=1):
+        import numpy as np
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        targets = np.ascontiguousarray(targets, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self.lib.mdr_mlp_train(self.ctx, _fptr(feats), _fptr(targets), len(feats), epochs, batch, lr,
+                                           seed, ctypes.byref(loss)), "mlp_train")
+        return loss.value
+
+
+for _name in dir(_CostModel):
+    if not _name.startswith("_"):
+        setattr(Engine, _name, getattr(_CostModel, _name))

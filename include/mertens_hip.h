@@ -69,9 +69,14 @@ typedef struct {
     int params;               /* MDR_PARAMS_*: reference parameters or the autotuned GPU split */
     int sched;                /* MDR_SCHED_* */
     int host_threads;         /* accepted for interface parity; the GPU path ignores it */
-    long long chunk;          /* static chunk length in positions, 0 = autotuned default */
-    int record_cycles;        /* 1: record per-chunk cycle counts (training data) */
+    long long chunk;          /* quotient-phase chunk count P, 0 = automatic */
+    int record_cycles;        /* s > 0: log cycles of every cost piece of 1 in s G evaluations */
     int no_table_reuse;       /* 1: rebuild the table even when the cached one fits */
+    /* explicit GPU split for MDR_PARAMS_TUNED (0 = from the tuning cache):
+     * sieve cutoff, dense-tier limit and S1/S2 split factor */
+    long long tune_u;
+    long long tune_ud;
+    double tune_cx;
 } mdr_opts;
 
 typedef struct {
@@ -84,13 +89,27 @@ typedef struct {
     /* GPU phase times in milliseconds (hipEvents on the engine's stream) */
     double table_ms, partition_ms, quotient_ms;
     int table_reused;
-    long long U, u_dense, mu_limit, W;   /* table actually used */
+    long long U, u_dense, mu_limit, W;   /* table actually used; W = compressed 64-byte lines */
     long long u, N, slots, chunks;       /* last X of the call */
     double cx;
     long long s1_terms, s2_terms;        /* quotient positions (sum over the call) */
     double table_bytes;                  /* bytes of the table arrays written */
     long long s1_comp_terms;             /* S1 positions answered by the compressed tier */
+    int partition_used;                  /* MDR_PARTITION_* actually applied (learned falls
+                                            back to static without a cost model) */
 } mdr_stats;
+
+/* One cycle-log record: a run of S1 (n) or S2 (j) positions of one G(y)
+ * evaluation processed by one wave, and the shader-clock cycles it took. */
+typedef struct {
+    long long y, start, end;   /* start/end: first and last n or j of the run */
+    unsigned int kind;         /* 0..3 S1 {dense,compressed}x{stride 1,2}; 4..15 S2 pieces 1..12 */
+    unsigned int cycles;
+    unsigned long long count;  /* positions */
+} mdr_stamp;
+
+#define MDR_MLP_NFEAT 5
+#define MDR_MLP_NPARAM 1281
 
 void mdr_default_opts(mdr_opts* opts);
 mdr_status mdr_init(mdr_ctx** ctx, int device);
@@ -110,7 +129,7 @@ mdr_status mdr_plan(mdr_ctx* ctx, long long X, const mdr_opts* opts, int* stream
                     long long* u_dense, long long* mu_limit, long long* N, double* cx);
 
 /* Component entry points (parity tests).  mdr_table_build sieves n = 1..U
- * with the dense tier up to u_dense (a multiple of 64 when below U) and packed
+ * with the dense tier up to u_dense (a multiple of 192 when below U) and packed
  * mu up to mu_limit; the reads copy M(first..first+count-1) and mu(...) to host. */
 mdr_status mdr_table_build(mdr_ctx* ctx, long long U, long long u_dense, long long mu_limit);
 mdr_status mdr_table_read_M(mdr_ctx* ctx, long long first, long long count, int32_t* out);
@@ -125,6 +144,29 @@ mdr_status mdr_s1_sum(mdr_ctx* ctx, long long y, long long first, long long last
 /* G(y) (comb2 = 0) or G(y) - G(floor(y/2)) (comb2 = 1) with cutoff u and split cx. */
 mdr_status mdr_g_eval(mdr_ctx* ctx, long long y, int comb2, long long u, double cx, long long* lo,
                       long long* hi);
+
+/* Learned cost model: a 5 -> 32 -> 32 -> 1 FP32 MLP (ReLU) predicting
+ * log2(cycles per position) of a quotient run; no reference counterpart (the
+ * reference schedules whole terms dynamically, mertens_dr.hpp:1616).
+ * Weights are MDR_MLP_NPARAM floats: W1[32][5], b1[32], W2[32][32], b2[32],
+ * W3[32], b3.  set with n = 0 removes the model (static partition). */
+mdr_status mdr_cost_model_set(mdr_ctx* ctx, const float* w, int n);
+mdr_status mdr_cost_model_get(mdr_ctx* ctx, float* w, int n);
+mdr_status mdr_cost_model_load(mdr_ctx* ctx, const char* path);
+mdr_status mdr_cost_model_save(mdr_ctx* ctx, const char* path);
+/* Cycle log of the last call made with opts.record_cycles = stride > 0. */
+mdr_status mdr_cycle_log(mdr_ctx* ctx, mdr_stamp* out, size_t cap, size_t* n);
+/* Features (n x 5) and targets log2(cycles/count) of cycle-log records (host). */
+mdr_status mdr_mlp_features(const mdr_stamp* s, size_t n, float* feats, float* targets);
+/* GPU kernels of the network: inference, one forward+backward pass (mean
+ * squared error over n samples; grad has MDR_MLP_NPARAM entries) and Adam
+ * training from the context's model (He-uniform from seed when it has none);
+ * training installs the result as the context's cost model. */
+mdr_status mdr_mlp_predict(mdr_ctx* ctx, const float* w, const float* feats, size_t n, float* out);
+mdr_status mdr_mlp_grad(mdr_ctx* ctx, const float* w, const float* feats, const float* targets, size_t n,
+                        float* grad, float* loss);
+mdr_status mdr_mlp_train(mdr_ctx* ctx, const float* feats, const float* targets, size_t n, int epochs,
+                         int batch, float lr, unsigned int seed, float* final_loss);
 
 #ifdef __cplusplus
 }

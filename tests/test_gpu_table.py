@@ -32,8 +32,8 @@ def cases():
     out = []
     for U in [1, 2, 3, 4, 63, 64, 65, 255, 256, 257, 13859, 13860, 13861, 17325, SEG - 1, SEG, SEG + 1,
               2 * SEG + 5, 277199, 277200, 277201, 1000033, LIM]:
-        for ud in sorted({U, 0, 64, 4096, (SEG // 64) * 64, (U // 128) * 64}):
-            if ud > U or (ud < U and ud % 64):
+        for ud in sorted({U, 0, 192, 4032, SEG, SEG + 192, (U // 384) * 192}):
+            if ud > U or (ud < U and ud % 192):
                 continue
             for ml in sorted({U, max(1, U // 3)}):
                 out.append((U, ud, ml))
@@ -53,7 +53,7 @@ def test_table_matches_eratosthenes(engine, mu_ref, pre_ref, U, ud, ml):
 def test_large_table_windows(engine, known, ref_values):
     # A 2e10 table (two tiers, ~6 GB): mu windows against trial division,
     # M differences against mu, and M at the known / reference-computed points.
-    U, ud = 20_000_000_000, 128_000_000
+    U, ud = 20_000_000_000, 128_000_064
     print("large table", U, flush=True)
     engine.table_build(U, ud, 60_000_000)
     for X in [10**9, 10**10]:
