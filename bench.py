@@ -103,13 +103,17 @@ def main():
     expect = known.get(X, ref.get(X))
 
     E = mdr_hip.Engine(0)
-    # learned and static partitions under the same persistent static round-robin schedule
+    # production path: learned cuts fed to the dynamic chunk queue
     learned = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
-                                sched=mdr_hip.SCHED_STATIC, no_table_reuse=1)
-    static = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
-                               sched=mdr_hip.SCHED_STATIC, no_table_reuse=1)
-    dynamic = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
                                 sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
+    # partition comparison under the static round-robin schedule (one chunk per wave)
+    nchunk = E.waves()
+    learned_rr = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
+                                   sched=mdr_hip.SCHED_STATIC, chunk=nchunk, no_table_reuse=1)
+    static_rr = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
+                                  sched=mdr_hip.SCHED_STATIC, chunk=nchunk, no_table_reuse=1)
+    static_q = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
+                                 sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
 
     def timed(opts, k, tag):
         res, stats = [], []
@@ -130,16 +134,22 @@ def main():
         log(f"warmup {i} done")
     t_learned, st_l = timed(learned, args.steps, "learned")
     used = "learned" if st_l[-1].partition_used == mdr_hip.PARTITION_LEARNED else "static (no cost model)"
-    t_static, st_s, t_dyn = None, None, None
+    cmp = None
     if not args.no_static:
-        t_static, st_s = timed(static, args.steps, "static")
-        t_dyn, _ = timed(dynamic, args.steps, "static+queue")
+        t_sq, _ = timed(static_q, args.steps, "static+queue")
+        t_lrr, _ = timed(learned_rr, args.steps, "learned+round-robin")
+        t_srr, _ = timed(static_rr, args.steps, "static+round-robin")
+        cmp = {"schedule_round_robin_chunks": nchunk,
+               "wall_s_learned_round_robin": round(t_lrr, 4), "wall_s_static_round_robin": round(t_srr, 4),
+               "static_over_learned_round_robin": round(t_srr / t_lrr, 3),
+               "wall_s_static_queue": round(t_sq, 4), "static_over_learned_queue": round(t_sq / t_learned, 3)}
+        log("partition comparison", cmp)
 
     batch = None
     if not args.no_batch:
         xs = load_json("xlists.json")["batch_4096_1e9_1e14_seed3"]
         bo = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
-                               sched=mdr_hip.SCHED_STATIC, no_table_reuse=1)
+                               sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
         E.batch(xs[:64], bo)
         t0 = time.perf_counter()
         out = E.batch(xs, bo)
@@ -149,7 +159,7 @@ def main():
         bad = sum(out[i] != ref[xs[i]] for i in checked)
         one = xs[:256]
         so = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
-                               sched=mdr_hip.SCHED_STATIC)
+                               sched=mdr_hip.SCHED_DYNAMIC)
         t0 = time.perf_counter()
         single = [E.mertens(x, so) for x in one]
         t1 = time.perf_counter() - t0
@@ -179,10 +189,8 @@ def main():
                    "parallelism": "1 GPU"},
         "result": E.mertens(X, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED)) if expect is None else expect,
         "wall_s_learned": round(t_learned, 4),
-        "wall_s_static": None if t_static is None else round(t_static, 4),
-        "static_over_learned": None if t_static is None else round(t_static / t_learned, 3),
-        "wall_s_static_dynamic_queue": None if t_dyn is None else round(t_dyn, 4),
         "partition_used": used,
+        "partition_comparison": cmp,
         "phase_ms": {"table": round(st.table_ms, 2), "partition": round(st.partition_ms, 2),
                      "quotient": round(st.quotient_ms, 2)},
         "quotient_positions": {"s1_dense": st.s1_terms - st.s1_comp_terms, "s1_comp": st.s1_comp_terms,

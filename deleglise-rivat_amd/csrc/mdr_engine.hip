@@ -643,6 +643,8 @@ void mdr_shutdown(mdr_ctx* c) {
 
 const char* mdr_last_error(mdr_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+long long mdr_quot_waves(mdr_ctx* c) { return c ? (long long)c->ncu * c->quot_blocks_per_cu * 4 : 0; }
+
 mdr_status mdr_plan(mdr_ctx* c, long long X, const mdr_opts* opts, int* streaming, long long* u,
                     long long* u_dense, long long* mu_limit, long long* N, double* cx) {
     // Pure host logic: ctx may be NULL (then errors carry no message).

@@ -156,13 +156,54 @@ __device__ i64 s1_dense(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane)
 template <int ST>
 __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) {
     const double dy = (double)y;
+    const u64 nmax = nlo + (u64)ST * (cnt - 1);
     i64 s = 0;
+    if (nmax < 1431655765ull) {
+        // n < 2^32 with 3n <= 2^32, q <= U < 2^44.  q0 = nearest integer to
+        // the estimate (2^52 + qd leaves it in the low mantissa bits); q0 is q
+        // or q + 1, so the true remainder y - q0*n lies in [-n, n) and its low
+        // 32 bits decide the correction exactly.
+        const uint32_t ylo = (uint32_t)y;
+        const uint32_t nb = (uint32_t)nlo;
+        const u64 base = (u64)tp.ud + 1;
+        for (uint32_t i = lane; i < (uint32_t)cnt; i += 64) {
+            const uint32_t n = nb + (uint32_t)ST * i;
+            const double qd = dy * rcp_refined((double)n);
+            const u64 q0 = (u64)__double_as_longlong(qd + 4503599627370496.0) & 0xfffffffffffffull;
+            const uint32_t r = ylo - (uint32_t)q0 * n;
+            const u64 q = q0 - (r >= 2u * n ? 1ull : 0ull);
+            s += comp_M(tp.cl, q - base);
+        }
+        return s;
+    }
     for (u64 i = lane; i < cnt; i += 64) {
         const u64 n = nlo + (u64)ST * i;
         const u64 q = qdiv_mid(y, dy, n, rcp_refined((double)n));
         s += comp_M(tp.cl, q - (u64)tp.ud - 1);
     }
     return s;
+}
+
+// S2 summand of a quotient q < 2^32 in 32-bit modular arithmetic: every
+// value lies in [0, 2^32) (pieces other than 4) or in (-2^29, 2^29) (piece 4),
+// so the wrapped 32-bit result is exact.
+template <int P>
+__device__ __forceinline__ uint32_t piece32(uint32_t q) {
+    const uint32_t h = q >> 1, f = q >> 2;
+    if constexpr (P == 6) return q & 1u;
+    else if constexpr (P == 7 || P == 11) return q - h;
+    else if constexpr (P == 8 || P == 12) return q;
+    else if constexpr (P == 5) return q - 2u * h + f;
+    else {
+        const uint32_t t = __umulhi(q, 0xaaaaaaabu) >> 1;   // q / 3
+        const uint32_t sx = t >> 1, w = t >> 2;              // q / 6, q / 12
+        if constexpr (P == 1) return q - 2u * h - t + 2u * sx + f - w;
+        else if constexpr (P == 2) return q - 2u * h - t + 2u * sx + f;
+        else if constexpr (P == 3) return q - 2u * h - t + f + sx;
+        else if constexpr (P == 4) return q - 2u * h - t + f;
+        else if constexpr (P == 9) return q - h - t + sx;
+        else return q - h - t;   // P == 10
+    }
 }
 
 template <int P>
@@ -182,18 +223,28 @@ __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane)
         const i64 v = piece_eval<P, u64>(q);
         s += code == 2 ? v : (code == 0 ? -v : 0);
     }
+    // q < 2^32: with code = mu + 1 in {0, 1, 2}, sum mu*v = sum code*v - sum v
+    // (one multiply-add per position); piece 4 can be negative and takes the
+    // signed form.
     const uint32_t ylo = (uint32_t)y;
     const uint32_t cnt = (uint32_t)(t_hi - ts);
     const uint32_t tb = (uint32_t)ts;
+    u64 acc_c = 0, acc_v = 0;
+    i64 acc_s = 0;
     for (uint32_t i = lane; i < cnt; i += 64) {
         const uint32_t t = tb + i;
         const uint32_t j = 3u * t + 1u + (t & 1u);
-        const int code = (int)((tp.mu6[t >> 4] >> (2 * (t & 15))) & 3u);
+        const uint32_t code = (tp.mu6[t >> 4] >> (2 * (t & 15))) & 3u;
         const uint32_t q = qdiv32(ylo, dy, j, rcp_refined((double)j));
-        const i64 v = piece_eval<P, uint32_t>(q);
-        s += code == 2 ? v : (code == 0 ? -v : 0);
+        const uint32_t v = piece32<P>(q);
+        if constexpr (P == 4) {
+            acc_s += (i64)((int)code - 1) * (i64)(int32_t)v;
+        } else {
+            acc_c += (u64)code * (u64)v;
+            acc_v += (u64)v;
+        }
     }
-    return s;
+    return s + (i64)(acc_c - acc_v) + acc_s;
 }
 
 __device__ i64 run_seg(const TableParam& tp, const Seg& sg, u64 y, u64 p_lo, u64 p_hi, int lane) {

@@ -25,7 +25,8 @@ EXPORTS = ["mdr_default_opts", "mdr_init", "mdr_shutdown", "mdr_last_error", "md
            "mdr_mertens_batch", "mdr_plan", "mdr_table_build", "mdr_table_read_M",
            "mdr_table_read_mu", "mdr_s2_interval", "mdr_s1_sum", "mdr_g_eval",
            "mdr_cost_model_set", "mdr_cost_model_get", "mdr_cost_model_load", "mdr_cost_model_save",
-           "mdr_cycle_log", "mdr_mlp_features", "mdr_mlp_predict", "mdr_mlp_grad", "mdr_mlp_train"]
+           "mdr_cycle_log", "mdr_mlp_features", "mdr_mlp_predict", "mdr_mlp_grad", "mdr_mlp_train",
+           "mdr_quot_waves"]
 
 MLP_NPARAM = 1281
 MLP_NFEAT = 5
@@ -105,6 +106,7 @@ def load():
     lib.mdr_s1_sum.restype = i32
     lib.mdr_g_eval.argtypes = [vp, ll, i32, ll, dbl, P(ll), P(ll)]
     lib.mdr_g_eval.restype = i32
+    lib.mdr_quot_waves.argtypes, lib.mdr_quot_waves.restype = [vp], ll
     fp = P(ctypes.c_float)
     sz = ctypes.c_size_t
     lib.mdr_cost_model_set.argtypes, lib.mdr_cost_model_set.restype = [vp, fp, i32], i32
@@ -191,6 +193,10 @@ class Engine:
         st = self.lib.mdr_init(ctypes.byref(self.ctx), device)
         if st != 0:
             raise MdrError(f"mdr_init failed: {STATUS.get(st, st)}")
+
+    def waves(self):
+        """Resident waves of the quotient kernel on this GPU."""
+        return self.lib.mdr_quot_waves(self.ctx)
 
     def close(self):
         if self.ctx:

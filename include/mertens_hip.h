@@ -123,6 +123,10 @@ mdr_status mdr_mertens(mdr_ctx* ctx, long long X, const mdr_opts* opts, long lon
 mdr_status mdr_mertens_batch(mdr_ctx* ctx, const long long* X, size_t n, const mdr_opts* opts,
                              long long* out, mdr_stats* stats);
 
+/* Waves the persistent quotient kernel keeps resident on the context's GPU
+ * (the natural chunk count of a static round-robin schedule). */
+long long mdr_quot_waves(mdr_ctx* ctx);
+
 /* Parameters the engine would use for X (reference rules or tuned).  Host-only;
  * ctx may be NULL. */
 mdr_status mdr_plan(mdr_ctx* ctx, long long X, const mdr_opts* opts, int* streaming, long long* u,
