@@ -87,8 +87,10 @@ struct mdr_ctx {
     int n_w = 0, n_small = 0, n_sq = 0, n_sq_small = 0;
     DevBuf stencil, wprimes, wrcp, wweights, sqprimes;
     // work
-    DevBuf xparam, slot_len, slot_cost, bounds, out128, ctr, stamps, comp_out;
+    DevBuf xparam, slot_len, slot_cost, bounds, out128, ctr, stamps, comp_out, lookup;
     std::vector<StampRec> last_stamps;
+    // instantiated graphs, one per launch sequence (see launch_graph)
+    hipGraphExec_t graph_exec[16] = {};
     // learned cost model
     DevBuf mlp;
     bool mlp_ok = false;
@@ -186,6 +188,7 @@ mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
     if (o.tune_u < 0 || o.tune_ud < 0 || o.tune_cx < 0 || (o.tune_cx != 0 && !(o.tune_cx > 0.1 && o.tune_cx < 5.0)))
         return fail(c, MDR_ERR_INVALID, "tuning override out of range");
     if (o.tune_u > 1000000000000LL) return fail(c, MDR_ERR_INVALID, "tune_u above 1e12");
+    if (o.use_graph < 0 || o.use_graph > 1) return fail(c, MDR_ERR_INVALID, "use_graph must be 0 or 1");
     return MDR_OK;
 }
 
@@ -291,35 +294,56 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
 }
 
 // ---- table -----------------------------------------------------------------------
-mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, bool reuse = true) {
+// A table request after validation: the sizes every kernel of the build sees.
+struct TableJob {
+    bool build = false;   // false: the cached table already covers the request
+    i64 U = 0, ud = 0, mu_lim = 0, NL = 0, nseg = 0, musegs = 0;
+};
+
+// Validates a table request and allocates what its build needs (host side:
+// everything that may allocate happens here, before any kernel is enqueued).
+mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, TableJob& j) {
     if (U < 1) U = 1;
     if (ud > U) ud = U;
     if (ud < U && (ud % LINE_VALUES) != 0)
         return fail(c, MDR_ERR_INVALID, "u_dense must be a multiple of 192 below U");
     if (mu_lim > U) return fail(c, MDR_ERR_INVALID, "mu_limit above U");
     if (mu_lim < 1) mu_lim = 1;
-    if (reuse && c->tab_valid && c->tU == U && c->tud == ud && c->tmu == mu_lim) {
-        if (stats) stats->table_reused = 1;
-        return MDR_OK;
-    }
+    j = TableJob{};
+    j.U = U;
+    j.ud = ud;
+    j.mu_lim = mu_lim;
+    j.nseg = (U + SEG - 1) / SEG;
+    j.musegs = (mu_lim + SEG - 1) / SEG;
+    j.NL = U > ud ? (U - ud + LINE_VALUES - 1) / LINE_VALUES : 0;
+    if (reuse && c->tab_valid && c->tU == U && c->tud == ud && c->tmu == mu_lim) return MDR_OK;
+    j.build = true;
     c->tab_valid = false;
     mdr_status s = ensure_primes(c, U);
     if (s != MDR_OK) return s;
-    const i64 nseg = (U + SEG - 1) / SEG;
-    const i64 musegs = (mu_lim + SEG - 1) / SEG;
-    const i64 NL = U > ud ? (U - ud + LINE_VALUES - 1) / LINE_VALUES : 0;
     HIPT(c->Md.ensure((size_t)(ud + 1) * 2), "alloc dense table");
-    HIPT(c->cl.ensure((size_t)NL * sizeof(CLine)), "alloc compressed lines");
-    HIPT(c->mu6.ensure((size_t)musegs * (SEG / 48) * 4), "alloc mu");
-    HIPT(c->segsum.ensure((size_t)nseg * 8), "alloc segment sums");
-    HIPT(c->scan_tmp.ensure((size_t)scan_tmp_len(nseg) * 8), "alloc scan");
-    HIPT(hipMemsetAsync(c->Md.p, 0, 2, c->st), "clear");
-    HIPT(hipEventRecord(c->ev[0], c->st), "event");
+    HIPT(c->cl.ensure((size_t)j.NL * sizeof(CLine)), "alloc compressed lines");
+    HIPT(c->mu6.ensure((size_t)j.musegs * (SEG / 48) * 4), "alloc mu");
+    HIPT(c->segsum.ensure((size_t)j.nseg * 8), "alloc segment sums");
+    HIPT(c->scan_tmp.ensure((size_t)scan_tmp_len(j.nseg) * 8), "alloc scan");
+    // the quotient kernels read the table through these from now on; the
+    // table counts as valid only once its build has completed
+    c->tU = U;
+    c->tud = ud;
+    c->tmu = mu_lim;
+    c->tL = j.NL;
+    c->tnseg = j.nseg;
+    c->tmusegs = j.musegs;
+    return MDR_OK;
+}
+
+// Enqueues the build of a prepared table: kernel launches only (capturable).
+mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     SieveArgs a;
-    a.U = U;
-    a.ud = ud;
-    a.L = NL;
-    a.mu_segs = musegs;
+    a.U = j.U;
+    a.ud = j.ud;
+    a.L = j.NL;
+    a.mu_segs = j.musegs;
     a.stencil = c->stencil.as<uint8_t>();
     a.wprimes = c->wprimes.as<uint32_t>();
     a.wrcp = c->wrcp.as<double>();
@@ -334,38 +358,51 @@ mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, 
     a.mu6 = c->mu6.as<uint32_t>();
     a.segsum = c->segsum.as<i64>();
     const i64 maxgrid = 1 << 30;
-    for (i64 s0 = 0; s0 < nseg; s0 += maxgrid) {
+    for (i64 s0 = 0; s0 < j.nseg; s0 += maxgrid) {
         a.seg_first = s0;
-        const i64 nb = std::min(maxgrid, nseg - s0);
+        const i64 nb = std::min(maxgrid, j.nseg - s0);
         hipLaunchKernelGGL(k_sieve, dim3((unsigned)nb), dim3(SIEVE_THREADS), 0, c->st, a);
     }
     HIPT(hipGetLastError(), "launch sieve");
-    HIPT(scan_exclusive_i64(c->segsum.as<i64>(), c->segsum.as<i64>(), nseg, c->scan_tmp.as<i64>(),
-                            scan_tmp_len(nseg), nullptr, c->st),
+    HIPT(scan_exclusive_i64(c->segsum.as<i64>(), c->segsum.as<i64>(), j.nseg, c->scan_tmp.as<i64>(),
+                            scan_tmp_len(j.nseg), nullptr, c->st),
          "scan segments");
     FixupArgs f;
-    f.ud = ud;
-    f.L = NL;
+    f.ud = j.ud;
+    f.L = j.NL;
     f.Md = c->Md.as<int16_t>();
     f.cl = c->cl.as<CLine>();
     f.segoff = c->segsum.as<i64>();
-    const i64 nfix = std::max(ud, NL);
-    if (nfix > 0) hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)((nfix + 255) / 256)), dim3(256), 0, c->st, f);
+    const i64 nfix = std::max<i64>(1, std::max(j.ud, j.NL));
+    hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)((nfix + 255) / 256)), dim3(256), 0, c->st, f);
     HIPT(hipGetLastError(), "launch fixup");
+    return MDR_OK;
+}
+
+double table_bytes(const TableJob& j) {
+    return 2.0 * (double)(j.ud + 1) + (double)sizeof(CLine) * (double)j.NL + (double)j.musegs * (SEG / 12);
+}
+
+// Builds (or reuses) the table and waits for it.
+mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, bool reuse = true) {
+    TableJob j;
+    mdr_status s = prepare_table(c, U, ud, mu_lim, reuse, j);
+    if (s != MDR_OK) return s;
+    if (!j.build) {
+        if (stats) stats->table_reused = 1;
+        return MDR_OK;
+    }
+    HIPT(hipEventRecord(c->ev[0], c->st), "event");
+    s = enqueue_table(c, j);
+    if (s != MDR_OK) return s;
     HIPT(hipEventRecord(c->ev[1], c->st), "event");
     HIPT(hipEventSynchronize(c->ev[1]), "table build");
-    c->tU = U;
-    c->tud = ud;
-    c->tmu = mu_lim;
-    c->tL = NL;
-    c->tnseg = nseg;
-    c->tmusegs = musegs;
     c->tab_valid = true;
     if (stats) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
         stats->table_ms += ms;
-        stats->table_bytes += 2.0 * (double)(ud + 1) + (double)sizeof(CLine) * (double)NL + (double)musegs * (SEG / 12);
+        stats->table_bytes += table_bytes(j);
     }
     return MDR_OK;
 }
@@ -378,15 +415,113 @@ __global__ void k_lookup(TableParam tp, const i64* qs, const int* xi, int n, u64
     out128[2 * xi[i] + 1] = (u64)(v >> 63);
 }
 
-// ---- quotient phase ------------------------------------------------------------------
+// ---- one call: table, lookups, partition, quotient phase -------------------------------
+// The call is prepared on the host (sizes, allocations, uploads), then enqueued
+// as kernel launches only, either straight onto the engine's stream or
+// captured into a hipGraph that is instantiated once per shape of the launch
+// sequence and updated in place (hipGraphExecUpdate) for later calls.
+struct BatchLaunch {
+    TableJob tj;
+    TableParam tp{};
+    size_t nx = 0, nxs = 0, nlook = 0;
+    i64 slots = 0, P = 0, cap = 0;
+    bool learned = false;
+    PartArgs pa{};
+    QuotArgs qa{};
+};
+
+// Kernel launches of one call (no allocation, no memset, no copy, no host
+// wait), so the same code runs eagerly and under stream capture.
+mdr_status enqueue_batch(mdr_ctx* c, const BatchLaunch& b, bool timed) {
+    unsigned long long* ctr = c->ctr.as<unsigned long long>();
+    if (timed) HIPT(hipEventRecord(c->ev[0], c->st), "event");
+    const i64 nclear = std::max<i64>(8, (i64)std::max(b.nx, b.nxs) * 2);
+    hipLaunchKernelGGL(k_clear3, dim3((unsigned)((nclear + 255) / 256)), dim3(256), 0, c->st, c->out128.as<u64>(),
+                       (i64)b.nx * 2, (u64*)ctr, (i64)8, c->comp_out.as<u64>(), (i64)b.nxs * 2);
+    HIPT(hipGetLastError(), "launch clear");
+    if (b.tj.build) {
+        mdr_status s = enqueue_table(c, b.tj);
+        if (s != MDR_OK) return s;
+    }
+    if (timed) HIPT(hipEventRecord(c->ev[1], c->st), "event");
+    if (b.nlook) {
+        const char* lk = static_cast<const char*>(c->lookup.p);
+        hipLaunchKernelGGL(k_lookup, dim3((unsigned)((b.nlook + 255) / 256)), dim3(256), 0, c->st, b.tp,
+                           (const i64*)lk, (const int*)(lk + b.nlook * 8), (int)b.nlook, c->out128.as<u64>());
+        HIPT(hipGetLastError(), "launch lookup");
+    }
+    if (b.nxs) {
+        if (timed) HIPT(hipEventRecord(c->ev[2], c->st), "event");
+        hipLaunchKernelGGL(k_slot_len, dim3((unsigned)((b.slots + 255) / 256)), dim3(256), 0, c->st, b.pa);
+        HIPT(hipGetLastError(), "launch slot lengths");
+        HIPT(scan_exclusive_i64(b.pa.len, b.pa.len, b.slots, c->scan_tmp.as<i64>(), scan_tmp_len(b.slots),
+                                (i64*)(ctr + 4), c->st),
+             "scan positions");
+        if (b.learned)
+            HIPT(scan_exclusive_i64(b.pa.cost, b.pa.cost, b.slots, c->scan_tmp.as<i64>(), scan_tmp_len(b.slots),
+                                    (i64*)(ctr + 5), c->st),
+                 "scan costs");
+        hipLaunchKernelGGL(k_cuts, dim3((unsigned)((b.P + 1 + 255) / 256)), dim3(256), 0, c->st, b.pa);
+        HIPT(hipGetLastError(), "launch cuts");
+        if (timed) HIPT(hipEventRecord(c->ev[3], c->st), "event");
+        const unsigned grid = (unsigned)(c->ncu * c->quot_blocks_per_cu);
+        hipLaunchKernelGGL(k_quot, dim3(grid), dim3(256), 0, c->st, b.qa);
+        HIPT(hipGetLastError(), "launch quotient kernel");
+    }
+    if (timed) HIPT(hipEventRecord(c->ev[4], c->st), "event");
+    return MDR_OK;
+}
+
+// Captures the call into a graph and launches it, reusing the instantiated
+// graph of the previous call with the same launch sequence.
+mdr_status launch_graph(mdr_ctx* c, const BatchLaunch& b, bool& instantiated) {
+    HIPT(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal), "begin capture");
+    const mdr_status s = enqueue_batch(c, b, false);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->st, &g);
+    if (s != MDR_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+    }
+    if (ec != hipSuccess) return hip_fail(c, ec, "end capture");
+    // one cached executable per launch sequence: with and without the table build,
+    // with and without the learned cost scan, with and without quotient work
+    const int key = (b.tj.build ? 1 : 0) | (b.learned ? 2 : 0) | (b.nxs ? 4 : 0) | (b.nlook ? 8 : 0);
+    hipGraphExec_t& ex = c->graph_exec[key];
+    instantiated = true;
+    if (ex) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult r = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(ex, g, &bad, &r) == hipSuccess && r == hipGraphExecUpdateSuccess) {
+            instantiated = false;
+        } else {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(ex);
+            ex = nullptr;
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (instantiated) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        ex = nullptr;
+        return hip_fail(c, e, "instantiate graph");
+    }
+    HIPT(hipEventRecord(c->ev[5], c->st), "event");
+    HIPT(hipGraphLaunch(ex, c->st), "launch graph");
+    HIPT(hipEventRecord(c->ev[6], c->st), "event");
+    return MDR_OK;
+}
+
 mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts& o, long long* out,
                      mdr_stats* stats) {
     const size_t nx = plans.size();
     // table covering every X
     i64 U = 1, mu_lim = 1, ud = 0;
-    bool any_sum = false;
+    bool any_sum = false, need_table = false;
     for (const XPlan& p : plans) {
         if (p.kind == 0) continue;
+        need_table = true;
         U = std::max(U, std::max(p.u, p.mu_lim));
         if (p.kind == 1) U = std::max(U, p.X);
         mu_lim = std::max(mu_lim, p.mu_lim);
@@ -398,14 +533,14 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
     // one dense tier for the batch: the largest any X asked for
     ud = any_sum ? std::min(U, ud) : U;
     if (ud < U) ud = (ud / LINE_VALUES) * LINE_VALUES;
-    const auto t_call = std::chrono::steady_clock::now();
-    bool need_table = false;
-    for (const XPlan& p : plans) need_table |= p.kind != 0;
+    BatchLaunch b;
+    b.nx = nx;
     if (need_table) {
-        mdr_status s = build_table(c, U, ud, mu_lim, stats, !o.no_table_reuse);
+        mdr_status s = prepare_table(c, U, ud, mu_lim, !o.no_table_reuse, b.tj);
         if (s != MDR_OK) return s;
+        if (stats && !b.tj.build) stats->table_reused = 1;
     }
-    const TableParam tp = c->tp();
+    b.tp = c->tp();
 
     std::vector<XParam> xps;
     std::vector<int> xidx;
@@ -430,137 +565,140 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
         xps.push_back(xp);
         xidx.push_back((int)i);
     }
+    b.nxs = xps.size();
+    b.nlook = lq.size();
+    b.slots = slots;
+
+    // ---- allocations and uploads: nothing after this block allocates ----
+    const auto th0 = std::chrono::steady_clock::now();
     HIPT(c->out128.ensure(nx * 16 + 16), "alloc results");
-    HIPT(hipMemsetAsync(c->out128.p, 0, nx * 16, c->st), "clear results");
-    if (!lq.empty()) {
-        DevBuf& b = c->comp_out;
-        HIPT(b.ensure(lq.size() * 12 + 16), "alloc lookups");
-        HIPT(hipMemcpyAsync(b.p, lq.data(), lq.size() * 8, hipMemcpyHostToDevice, c->st), "upload");
-        HIPT(hipMemcpyAsync((char*)b.p + lq.size() * 8, lx.data(), lx.size() * 4, hipMemcpyHostToDevice, c->st),
+    HIPT(c->ctr.ensure(64), "alloc counters");
+    HIPT(c->comp_out.ensure(b.nxs * 16 + 16), "alloc accumulators");
+    if (b.nlook) {
+        HIPT(c->lookup.ensure(b.nlook * 12 + 16), "alloc lookups");
+        HIPT(hipMemcpyAsync(c->lookup.p, lq.data(), b.nlook * 8, hipMemcpyHostToDevice, c->st), "upload");
+        HIPT(hipMemcpyAsync((char*)c->lookup.p + b.nlook * 8, lx.data(), b.nlook * 4, hipMemcpyHostToDevice, c->st),
              "upload");
-        hipLaunchKernelGGL(k_lookup, dim3((unsigned)((lq.size() + 255) / 256)), dim3(256), 0, c->st, tp,
-                           (const i64*)b.p, (const int*)((char*)b.p + lq.size() * 8), (int)lq.size(),
-                           c->out128.as<u64>());
-        HIPT(hipGetLastError(), "launch lookup");
     }
-    i64 P = 0;
-    if (!xps.empty()) {
-        HIPT(c->xparam.ensure(xps.size() * sizeof(XParam)), "alloc params");
-        const auto th0 = std::chrono::steady_clock::now();
-        HIPT(hipMemcpyAsync(c->xparam.p, xps.data(), xps.size() * sizeof(XParam), hipMemcpyHostToDevice, c->st),
-             "upload params");
-        HIPT(hipStreamSynchronize(c->st), "upload params");
-        if (stats) stats->h2d_copy_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
-        const bool learned = o.partition == MDR_PARTITION_LEARNED && c->mlp_ok;
+    if (b.nxs) {
+        b.learned = o.partition == MDR_PARTITION_LEARNED && c->mlp_ok;
         const i64 nwaves = (i64)c->ncu * c->quot_blocks_per_cu * 4;
         i64 tuned_chunks = 0;
-        if (o.params == MDR_PARAMS_TUNED) tuned_chunks = tune_chunks(plans.empty() ? 1 : plans.back().X);
-        P = o.chunk > 0 ? o.chunk : tuned_chunks > 0 ? tuned_chunks : nwaves * (o.sched == MDR_SCHED_DYNAMIC ? 16 : 4);
+        if (o.params == MDR_PARAMS_TUNED) tuned_chunks = tune_chunks(plans.back().X);
+        b.P = o.chunk > 0 ? o.chunk
+              : tuned_chunks > 0 ? tuned_chunks
+                                 : nwaves * (o.sched == MDR_SCHED_DYNAMIC ? 16 : 4);
+        HIPT(c->xparam.ensure(b.nxs * sizeof(XParam)), "alloc params");
         HIPT(c->slot_len.ensure((size_t)slots * 8), "alloc slot lengths");
-        if (learned) HIPT(c->slot_cost.ensure((size_t)slots * 8), "alloc slot costs");
-        HIPT(c->bounds.ensure((size_t)(P + 1) * 8), "alloc chunk bounds");
-        HIPT(c->scan_tmp.ensure((size_t)std::max(scan_tmp_len(slots), scan_tmp_len(c->tnseg)) * 8), "alloc scan");
-        HIPT(c->ctr.ensure(64), "alloc counters");
-        HIPT(hipMemsetAsync(c->ctr.p, 0, 64, c->st), "clear counters");
+        if (b.learned) HIPT(c->slot_cost.ensure((size_t)slots * 8), "alloc slot costs");
+        HIPT(c->bounds.ensure((size_t)(b.P + 1) * 8), "alloc chunk bounds");
+        HIPT(c->scan_tmp.ensure((size_t)std::max(scan_tmp_len(slots), scan_tmp_len(b.tj.nseg)) * 8), "alloc scan");
+        b.cap = o.record_cycles ? (i64)1 << 22 : 0;
+        if (b.cap) HIPT(c->stamps.ensure((size_t)b.cap * sizeof(StampRec)), "alloc cycle log");
+        HIPT(hipMemcpyAsync(c->xparam.p, xps.data(), b.nxs * sizeof(XParam), hipMemcpyHostToDevice, c->st),
+             "upload params");
         // ctr layout: [0] work queue, [1..3] position counts, [4] total positions,
         // [5] total predicted cycles, [6] cycle-log count
         unsigned long long* ctr = c->ctr.as<unsigned long long>();
-        PartArgs pa{};
+        PartArgs& pa = b.pa;
         pa.xp = c->xparam.as<XParam>();
-        pa.nx = (int)xps.size();
+        pa.nx = (int)b.nxs;
         pa.total_slots = slots;
-        pa.tp = tp;
-        pa.mlp = learned ? c->mlp.as<float>() : nullptr;
+        pa.tp = b.tp;
+        pa.mlp = b.learned ? c->mlp.as<float>() : nullptr;
         pa.len = c->slot_len.as<i64>();
-        pa.cost = learned ? c->slot_cost.as<i64>() : nullptr;
+        pa.cost = b.learned ? c->slot_cost.as<i64>() : nullptr;
         pa.type_pos = ctr + 1;
         pa.totals = reinterpret_cast<const i64*>(ctr + 4);
-        pa.P = P;
+        pa.P = b.P;
         pa.bounds = c->bounds.as<i64>();
-        HIPT(hipEventRecord(c->ev[2], c->st), "event");
-        const unsigned gslots = (unsigned)((slots + 255) / 256);
-        hipLaunchKernelGGL(k_slot_len, dim3(gslots), dim3(256), 0, c->st, pa);
-        HIPT(hipGetLastError(), "launch slot lengths");
-        const i64 nt = (slots + 2047) / 2048;
-        HIPT(scan_exclusive_i64(pa.len, pa.len, slots, c->scan_tmp.as<i64>(), scan_tmp_len(slots), nullptr, c->st),
-             "scan positions");
-        HIPT(hipMemcpyAsync(ctr + 4, c->scan_tmp.as<i64>() + nt, 8, hipMemcpyDeviceToDevice, c->st), "total");
-        if (learned) {
-            HIPT(scan_exclusive_i64(pa.cost, pa.cost, slots, c->scan_tmp.as<i64>(), scan_tmp_len(slots), nullptr,
-                                    c->st),
-                 "scan costs");
-            HIPT(hipMemcpyAsync(ctr + 5, c->scan_tmp.as<i64>() + nt, 8, hipMemcpyDeviceToDevice, c->st), "total");
-        }
-        hipLaunchKernelGGL(k_cuts, dim3((unsigned)((P + 1 + 255) / 256)), dim3(256), 0, c->st, pa);
-        HIPT(hipGetLastError(), "launch cuts");
-        // stamps (training data of the cost model)
-        const i64 cap = o.record_cycles ? (i64)1 << 22 : 0;
-        if (cap) HIPT(c->stamps.ensure((size_t)cap * sizeof(StampRec)), "alloc cycle log");
-        QuotArgs qa{};
+        QuotArgs& qa = b.qa;
         qa.xp = pa.xp;
         qa.nx = pa.nx;
-        qa.tp = tp;
+        qa.tp = b.tp;
         qa.off = pa.len;
         qa.nslots = slots;
         qa.bounds = pa.bounds;
-        qa.P = P;
+        qa.P = b.P;
         qa.next = ctr;
         qa.dynamic = o.sched == MDR_SCHED_DYNAMIC ? 1 : 0;
-        qa.stamps = cap ? c->stamps.as<StampRec>() : nullptr;
+        qa.stamps = b.cap ? c->stamps.as<StampRec>() : nullptr;
         qa.nstamps = ctr + 6;
-        qa.stamp_cap = cap;
+        qa.stamp_cap = b.cap;
         qa.stamp_stride = o.record_cycles > 1 ? o.record_cycles : 1;
-        HIPT(c->comp_out.ensure(xps.size() * 16 + 16), "alloc accumulators");
-        HIPT(hipMemsetAsync(c->comp_out.p, 0, xps.size() * 16, c->st), "clear accumulators");
         qa.out128 = c->comp_out.as<u64>();
-        const unsigned grid = (unsigned)(c->ncu * c->quot_blocks_per_cu);
-        HIPT(hipEventRecord(c->ev[3], c->st), "event");
-        hipLaunchKernelGGL(k_quot, dim3(grid), dim3(256), 0, c->st, qa);
-        HIPT(hipGetLastError(), "launch quotient kernel");
-        HIPT(hipEventRecord(c->ev[4], c->st), "event");
-        HIPT(hipEventSynchronize(c->ev[4]), "quotient phase");
-        std::vector<u64> acc_h(xps.size() * 2);
-        HIPT(hipMemcpy(acc_h.data(), c->comp_out.p, xps.size() * 16, hipMemcpyDeviceToHost), "download");
-        std::vector<u64> res(nx * 2);
-        HIPT(hipMemcpy(res.data(), c->out128.p, nx * 16, hipMemcpyDeviceToHost), "download");
-        for (size_t i = 0; i < xps.size(); ++i) {
+    }
+    if (stats) stats->h2d_copy_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
+
+    // ---- launch ----
+    bool instantiated = false;
+    if (o.use_graph) {
+        mdr_status s = launch_graph(c, b, instantiated);
+        if (s != MDR_OK) return s;
+    } else {
+        mdr_status s = enqueue_batch(c, b, true);
+        if (s != MDR_OK) return s;
+    }
+    HIPT(hipStreamSynchronize(c->st), "batch");
+    if (b.tj.build) c->tab_valid = true;
+
+    // ---- results ----
+    const auto td0 = std::chrono::steady_clock::now();
+    std::vector<u64> res(nx * 2);
+    if (nx) HIPT(hipMemcpy(res.data(), c->out128.p, nx * 16, hipMemcpyDeviceToHost), "download");
+    unsigned long long ch[7] = {};
+    if (b.nxs) {
+        std::vector<u64> acc_h(b.nxs * 2);
+        HIPT(hipMemcpy(acc_h.data(), c->comp_out.p, b.nxs * 16, hipMemcpyDeviceToHost), "download");
+        for (size_t i = 0; i < b.nxs; ++i) {
             res[2 * xidx[i]] = acc_h[2 * i];
             res[2 * xidx[i] + 1] = acc_h[2 * i + 1];
         }
-        for (size_t i = 0; i < nx; ++i) {
-            if (plans[i].kind == 0) { out[i] = plans[i].value; continue; }
-            const i64 lo = (i64)res[2 * i], hi = (i64)res[2 * i + 1];
-            if (hi != (lo >> 63)) return fail(c, MDR_ERR_RANGE, "result does not fit in 64 bits");
-            out[i] = lo;
-        }
-        unsigned long long ch[7];
         HIPT(hipMemcpy(ch, c->ctr.p, sizeof ch, hipMemcpyDeviceToHost), "download");
-        if (stats) {
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
-            stats->partition_ms += ms;
-            (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]);
-            stats->quotient_ms += ms;
-            stats->chunks += P;
+    }
+    for (size_t i = 0; i < nx; ++i) {
+        if (plans[i].kind == 0) { out[i] = plans[i].value; continue; }
+        const i64 lo = (i64)res[2 * i], hi = (i64)res[2 * i + 1];
+        if (hi != (lo >> 63)) return fail(c, MDR_ERR_RANGE, "result does not fit in 64 bits");
+        out[i] = lo;
+    }
+    if (b.cap) {
+        const i64 n = std::min<i64>((i64)ch[6], b.cap);
+        c->last_stamps.resize((size_t)n);
+        if (n) HIPT(hipMemcpy(c->last_stamps.data(), c->stamps.p, (size_t)n * sizeof(StampRec), hipMemcpyDeviceToHost),
+                    "download cycle log");
+    }
+    if (stats) {
+        stats->d2h_copy_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - td0).count();
+        float ms = 0;
+        if (o.use_graph) {
+            (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+            stats->graph_ms += ms;
+            stats->graph_used = 1;
+            stats->graph_instantiated += instantiated ? 1 : 0;
+            stats->kernel_time += ms / 1000.0;
+        } else {
+            if (b.tj.build) {
+                (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+                stats->table_ms += ms;
+            }
+            if (b.nxs) {
+                (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+                stats->partition_ms += ms;
+                (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]);
+                stats->quotient_ms += ms;
+            }
+            stats->kernel_time = (stats->table_ms + stats->partition_ms + stats->quotient_ms) / 1000.0;
+        }
+        if (b.tj.build) stats->table_bytes += table_bytes(b.tj);
+        if (b.nxs) {
+            stats->chunks += b.P;
             stats->slots += slots;
             stats->s1_terms += (long long)(ch[1] + ch[2]);
             stats->s1_comp_terms += (long long)ch[2];
             stats->s2_terms += (long long)ch[3];
-            stats->partition_used = learned ? MDR_PARTITION_LEARNED : MDR_PARTITION_STATIC;
+            stats->partition_used = b.learned ? MDR_PARTITION_LEARNED : MDR_PARTITION_STATIC;
         }
-        if (cap) {
-            const i64 n = std::min<i64>((i64)ch[6], cap);
-            c->last_stamps.resize((size_t)n);
-            if (n) HIPT(hipMemcpy(c->last_stamps.data(), c->stamps.p, (size_t)n * sizeof(StampRec), hipMemcpyDeviceToHost),
-                        "download cycle log");
-        }
-    } else {
-        HIPT(hipStreamSynchronize(c->st), "lookups");
-        std::vector<u64> res(nx * 2);
-        HIPT(hipMemcpy(res.data(), c->out128.p, nx * 16, hipMemcpyDeviceToHost), "download");
-        for (size_t i = 0; i < nx; ++i) out[i] = plans[i].kind == 0 ? plans[i].value : (i64)res[2 * i];
-    }
-    if (stats) {
         stats->U = c->tU;
         stats->u_dense = c->tud;
         stats->mu_limit = c->tmu;
@@ -571,8 +709,6 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
             stats->N = p.N;
             stats->cx = p.cx;
         }
-        stats->kernel_time = (stats->table_ms + stats->partition_ms + stats->quotient_ms) / 1000.0;
-        (void)t_call;
     }
     return MDR_OK;
 }
@@ -633,7 +769,9 @@ void mdr_shutdown(mdr_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     DevBuf* bufs[] = {&c->Md, &c->cl, &c->mu6, &c->segsum, &c->scan_tmp, &c->stencil, &c->wprimes,
                       &c->wrcp, &c->wweights, &c->sqprimes, &c->xparam, &c->slot_len, &c->slot_cost, &c->bounds,
-                      &c->out128, &c->ctr, &c->stamps, &c->comp_out, &c->mlp};
+                      &c->out128, &c->ctr, &c->stamps, &c->comp_out, &c->lookup, &c->mlp};
+    for (auto& g : c->graph_exec)
+        if (g) (void)hipGraphExecDestroy(g);
     for (DevBuf* b : bufs) b->release();
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);

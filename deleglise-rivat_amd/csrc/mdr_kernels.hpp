@@ -73,10 +73,16 @@ __global__ void k_slot_len(PartArgs a);
 __global__ void k_cuts(PartArgs a);
 __global__ void k_quot(QuotArgs a);
 
-// Exclusive scan of an i64 array (in place allowed).  The total lands in
-// d_tmp[(n + 2047) / 2048] on the device and, when total != nullptr, on the host.
-hipError_t scan_exclusive_i64(const i64* in, i64* out, i64 n, i64* d_tmp, i64 tmp_len, i64* total,
+// Exclusive scan of an i64 array (in place allowed), three kernel launches
+// for every n (n = 0 included), so that captured graphs keep one topology.
+// The total lands in d_tmp[max(1, (n + 2047) / 2048)] and, when dev_total is
+// not null, in *dev_total (device memory).
+hipError_t scan_exclusive_i64(const i64* in, i64* out, i64 n, i64* d_tmp, i64 tmp_len, i64* dev_total,
                               hipStream_t st);
+
+// Sets n0 words at p0, n1 at p1 and n2 at p2 to zero (a kernel, not a memset
+// node, so that it replays correctly inside captured graphs).
+__global__ void k_clear3(u64* p0, i64 n0, u64* p1, i64 n1, u64* p2, i64 n2);
 i64 scan_tmp_len(i64 n);
 
 // Cost-model network (mdr_mlp.hip).

@@ -412,7 +412,7 @@ __global__ void k_scan_reduce(const i64* in, i64 n, i64* tsum) {
     if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_top(i64* tsum, i64 nt, i64* total) {
+__global__ void k_scan_top(i64* tsum, i64 nt, i64* total, i64* total2) {
     i64 carry = 0;
     for (i64 b = 0; b < nt; b += SCAN_TILE) {
         const i64 base = b + threadIdx.x * SCAN_PER;
@@ -429,7 +429,10 @@ __global__ void k_scan_top(i64* tsum, i64 nt, i64* total) {
         }
         carry += tot;
     }
-    if (threadIdx.x == 0) *total = carry;
+    if (threadIdx.x == 0) {
+        *total = carry;
+        if (total2) *total2 = carry;
+    }
 }
 
 __global__ void k_scan_down(const i64* in, i64* out, i64 n, const i64* tsum) {
@@ -447,28 +450,25 @@ __global__ void k_scan_down(const i64* in, i64* out, i64 n, const i64* tsum) {
     }
 }
 
-i64 scan_tmp_len(i64 n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+__global__ void k_clear3(u64* p0, i64 n0, u64* p1, i64 n1, u64* p2, i64 n2) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n0) p0[i] = 0;
+    if (i < n1) p1[i] = 0;
+    if (i < n2) p2[i] = 0;
+}
 
-// d_tmp must hold scan_tmp_len(n) values; d_tmp[last] receives the total.
-hipError_t scan_exclusive_i64(const i64* in, i64* out, i64 n, i64* d_tmp, i64 tmp_len, i64* total,
+i64 scan_tmp_len(i64 n) { return (std::max<i64>(n, 1) + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+// d_tmp must hold scan_tmp_len(n) values; d_tmp[last] receives the total, and
+// so does *dev_total when it is not null.  Only kernel launches: capturable.
+hipError_t scan_exclusive_i64(const i64* in, i64* out, i64 n, i64* d_tmp, i64 tmp_len, i64* dev_total,
                               hipStream_t st) {
-    if (n <= 0) {
-        if (total) *total = 0;
-        return hipSuccess;
-    }
-    const i64 nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const i64 nt = (std::max<i64>(n, 1) + SCAN_TILE - 1) / SCAN_TILE;
     if (tmp_len < nt + 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(SCAN_T), 0, st, in, n, d_tmp);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, d_tmp, nt, d_tmp + nt);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, st, d_tmp, nt, d_tmp + nt, dev_total);
     hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nt), dim3(SCAN_T), 0, st, in, out, n, d_tmp);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (total) {
-        e = hipMemcpyAsync(total, d_tmp + nt, sizeof(i64), hipMemcpyDeviceToHost, st);
-        if (e != hipSuccess) return e;
-        e = hipStreamSynchronize(st);
-    }
-    return e;
+    return hipGetLastError();
 }
 
 }  // namespace mdr

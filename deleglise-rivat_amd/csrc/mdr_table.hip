@@ -261,9 +261,11 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
 }
 
-// Adds the exclusive segment offsets to the segment-local values.
+// Adds the exclusive segment offsets to the segment-local values and sets
+// M(0) = 0.
 __global__ void k_table_fixup(FixupArgs a) {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) a.Md[0] = 0;
     if (i < a.ud) {
         const i64 n = i + 1;
         const i64 s = (n - 1) / SEG;

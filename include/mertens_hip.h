@@ -77,6 +77,8 @@ typedef struct {
     long long tune_u;
     long long tune_ud;
     double tune_cx;
+    int use_graph;            /* 1: capture the call's launches into a hipGraph and replay it
+                                 (instantiated once per launch sequence, updated in place) */
 } mdr_opts;
 
 typedef struct {
@@ -97,6 +99,10 @@ typedef struct {
     long long s1_comp_terms;             /* S1 positions answered by the compressed tier */
     int partition_used;                  /* MDR_PARTITION_* actually applied (learned falls
                                             back to static without a cost model) */
+    int graph_used;                      /* the call ran as one hipGraph launch */
+    int graph_instantiated;              /* graphs instantiated (not updated in place) */
+    double graph_ms;                     /* GPU time of the graph launches; the per-phase
+                                            times stay 0 for graph calls */
 } mdr_stats;
 
 /* One cycle-log record: a run of S1 (n) or S2 (j) positions of one G(y)
