@@ -69,10 +69,14 @@ def cpu_baseline(threads):
             "seconds_1e16": t, "est_seconds_1e17": est17}
 
 
-def roofline(st):
+def roofline(stats):
+    """k_quot over the timed steps: algorithmic bytes per launch / mean launch
+    time (hipEvents recorded around the launch on the engine's stream)."""
+    st = stats[-1]
     byts = (BYTES_S1_DENSE * (st.s1_terms - st.s1_comp_terms) + BYTES_S1_COMP * st.s1_comp_terms
             + BYTES_S2 * st.s2_terms)
-    sec = st.quotient_ms / 1000.0
+    ms = sum(s.quotient_ms for s in stats) / len(stats)
+    sec = ms / 1000.0
     ach = byts / sec / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_k_quot_1e17.json")
@@ -80,7 +84,7 @@ def roofline(st):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
     return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_quot",
-            "algorithmic_bytes": byts, "kernel_ms": st.quotient_ms}
+            "algorithmic_bytes": byts, "kernel_ms": round(ms, 3), "launches": len(stats)}
 
 
 def main():
@@ -148,25 +152,38 @@ def main():
     batch = None
     if not args.no_batch:
         xs = load_json("xlists.json")["batch_4096_1e9_1e14_seed3"]
-        bo = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
-                               sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
+        kw = dict(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED, sched=mdr_hip.SCHED_DYNAMIC)
+        bo = mdr_hip.make_opts(no_table_reuse=1, **kw)
+        bg = mdr_hip.make_opts(no_table_reuse=1, use_graph=1, **kw)
         E.batch(xs[:64], bo)
+        E.batch(xs[:64], bg)
         t0 = time.perf_counter()
         out = E.batch(xs, bo)
         tb = time.perf_counter() - t0
         bst = E.last_stats
+        t0 = time.perf_counter()
+        outg = E.batch(xs, bg)
+        tg = time.perf_counter() - t0
         checked = [i for i, x in enumerate(xs) if x in ref]
-        bad = sum(out[i] != ref[xs[i]] for i in checked)
+        bad = sum(out[i] != ref[xs[i]] for i in checked) + sum(a != b for a, b in zip(out, outg))
+        # one-at-a-time: the same X values, one independent call each (own table,
+        # eager and graph launches)
         one = xs[:256]
-        so = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
-                               sched=mdr_hip.SCHED_DYNAMIC)
+        so = mdr_hip.make_opts(no_table_reuse=1, **kw)
+        sg = mdr_hip.make_opts(no_table_reuse=1, use_graph=1, **kw)
         t0 = time.perf_counter()
         single = [E.mertens(x, so) for x in one]
         t1 = time.perf_counter() - t0
-        bad += sum(single[i] != out[i] for i in range(len(one)))
-        batch = {"n": len(xs), "seconds": round(tb, 4), "evals_per_s": round(len(xs) / tb, 1),
+        t0 = time.perf_counter()
+        single_g = [E.mertens(x, sg) for x in one]
+        t1g = time.perf_counter() - t0
+        bad += sum(single[i] != out[i] for i in range(len(one))) + sum(single_g[i] != out[i] for i in range(len(one)))
+        batch = {"workload": "4096 X log-uniform in [1e9, 1e14], seed 3", "n": len(xs), "seconds": round(tb, 4),
+                 "evals_per_s": round(len(xs) / tb, 1),
+                 "graph_seconds": round(tg, 4), "graph_evals_per_s": round(len(xs) / tg, 1),
                  "one_at_a_time_evals_per_s": round(len(one) / t1, 1),
-                 "speedup_vs_one_at_a_time": round((len(xs) / tb) / (len(one) / t1), 2),
+                 "one_at_a_time_graph_evals_per_s": round(len(one) / t1g, 1),
+                 "speedup_vs_one_at_a_time": round((len(xs) / min(tb, tg)) / (len(one) / min(t1, t1g)), 2),
                  "checked_vs_reference": len(checked), "mismatches": int(bad),
                  "table_U": bst.U, "table_ms": round(bst.table_ms, 2), "quotient_ms": round(bst.quotient_ms, 2)}
         log("batch", batch)
@@ -179,6 +196,7 @@ def main():
         log("cpu", cpu)
 
     st = st_l[-1]
+    t_table = sum(s.table_ms for s in st_l) / len(st_l)
     line = {
         "metric": METRIC, "value": round(1.0 / t_learned, 5), "unit": "M(X)/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_learned * 1000, 2),
@@ -195,11 +213,11 @@ def main():
                      "quotient": round(st.quotient_ms, 2)},
         "quotient_positions": {"s1_dense": st.s1_terms - st.s1_comp_terms, "s1_comp": st.s1_comp_terms,
                                "s2": st.s2_terms},
-        "roofline": roofline(st),
-        "roofline_table": {"bound": "hbm", "achieved": round(st.table_bytes / (st.table_ms / 1000) / 1e9, 2),
+        "roofline": roofline(st_l),
+        "roofline_table": {"bound": "hbm", "achieved": round(st.table_bytes / (t_table / 1000) / 1e9, 2),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(st.table_bytes / (st.table_ms / 1000) / 1e9 / HBM_PEAK_GBS, 4),
-                           "bytes": st.table_bytes},
+                           "frac": round(st.table_bytes / (t_table / 1000) / 1e9 / HBM_PEAK_GBS, 4),
+                           "bytes": st.table_bytes, "ms": round(t_table, 3)},
         "batch": batch,
         "cpu_baseline": cpu,
     }

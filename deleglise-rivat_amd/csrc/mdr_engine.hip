@@ -240,6 +240,39 @@ XPlan plan_x(i64 X, const mdr_opts& o) {
     return p;
 }
 
+// Tuned batches share one cutoff: every X above it is summed with u = that
+// cutoff (M(X) does not depend on u), every X at or below it becomes a table
+// lookup.  Calls with one summed X, explicit overrides or the reference
+// parameters keep their per-X plans.
+void share_batch_cutoff(std::vector<XPlan>& plans, const mdr_opts& o) {
+    if (o.params != MDR_PARAMS_TUNED || o.tune_u > 0) return;
+    double xsum = 0;
+    size_t nsum = 0;
+    i64 umax = 0;
+    for (const XPlan& p : plans)
+        if (p.kind == 2) {
+            xsum += (double)p.X;
+            ++nsum;
+            umax = std::max(umax, p.u);
+        }
+    if (nsum < 2) return;
+    const i64 ub = tune_batch_u(xsum);
+    if (ub <= umax) return;
+    for (XPlan& p : plans) {
+        if (p.kind != 2 || p.u >= ub) continue;
+        if (p.X <= ub) {
+            p.kind = 1;
+            p.u = p.X;
+            p.mu_lim = 1;
+            continue;
+        }
+        p.u = ub;
+        p.N = p.X / p.u;
+        const i64 A2 = (i64)(p.cx * std::sqrt((double)p.X)) + 1000;
+        p.mu_lim = std::max(A2, p.N + 1000);
+    }
+}
+
 // ---- sieving primes and stencil ------------------------------------------------
 mdr_status ensure_primes(mdr_ctx* c, i64 U) {
     const i64 root = isqrt_host(U) + 1;
@@ -513,7 +546,7 @@ mdr_status launch_graph(mdr_ctx* c, const BatchLaunch& b, bool& instantiated) {
     return MDR_OK;
 }
 
-mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts& o, long long* out,
+mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, long long* out,
                      mdr_stats* stats) {
     const size_t nx = plans.size();
     // table covering every X
@@ -524,15 +557,37 @@ mdr_status run_batch(mdr_ctx* c, const std::vector<XPlan>& plans, const mdr_opts
         need_table = true;
         U = std::max(U, std::max(p.u, p.mu_lim));
         if (p.kind == 1) U = std::max(U, p.X);
-        mu_lim = std::max(mu_lim, p.mu_lim);
         if (p.kind == 2) {
             any_sum = true;
             ud = std::max(ud, p.ud);
+            mu_lim = std::max(mu_lim, p.mu_lim);
         }
     }
-    // one dense tier for the batch: the largest any X asked for
-    ud = any_sum ? std::min(U, ud) : U;
+    // one dense tier for the batch: the largest any X asked for; lookups
+    // alone keep a dense table only up to 5e7 (where |M| fits the int16 tier)
+    ud = any_sum ? std::min(U, ud) : U <= 50000000LL ? U : 1048512;
     if (ud < U) ud = (ud / LINE_VALUES) * LINE_VALUES;
+    // tuned calls: a cached table that covers the request serves it, and every
+    // summed X takes the cached cutoff (M(X) does not depend on u)
+    if (need_table && !o.no_table_reuse && c->tab_valid && o.params == MDR_PARAMS_TUNED && o.tune_u == 0 &&
+        c->tU >= U && c->tmu >= mu_lim) {
+        for (XPlan& p : plans) {
+            if (p.kind != 2 || p.u >= c->tU) continue;
+            if (p.X <= c->tU) {
+                p.kind = 1;
+                p.u = p.X;
+                p.mu_lim = 1;
+                continue;
+            }
+            p.u = c->tU;
+            p.N = p.X / p.u;
+            const i64 A2 = (i64)(p.cx * std::sqrt((double)p.X)) + 1000;
+            p.mu_lim = std::max(A2, p.N + 1000);
+        }
+        U = c->tU;
+        ud = c->tud;
+        mu_lim = c->tmu;
+    }
     BatchLaunch b;
     b.nx = nx;
     if (need_table) {
@@ -819,6 +874,7 @@ mdr_status mdr_mertens_batch(mdr_ctx* c, const long long* X, size_t n, const mdr
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
     std::vector<XPlan> plans(n);
     for (size_t i = 0; i < n; ++i) plans[i] = plan_x(X[i], o);
+    share_batch_cutoff(plans, o);
     s = run_batch(c, plans, o, out, stats);
     if (stats) stats->total_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return s;

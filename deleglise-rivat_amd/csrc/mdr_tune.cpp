@@ -3,6 +3,7 @@
 // The cache is a text file written by the project's autotuner
 // (deleglise-rivat_amd/autotune.py): one line per decade of X,
 //     decade  fx  cx  chunks  u_dense
+// and optionally one line "batch fb" (the batch cutoff factor),
 // where u = fx * (X / ln ln X)^(2/3) (the reference's cutoff form,
 // mertens_dr.hpp:1401-1419, with a GPU-specific factor), chunks is the
 // quotient-phase chunk count (0 = automatic) and u_dense the dense-tier
@@ -33,6 +34,7 @@ struct Entry {
 
 std::once_flag g_once;
 std::vector<Entry> g_entries;
+double g_batch_factor = 1.0;
 
 void load() {
     std::string path;
@@ -53,6 +55,11 @@ void load() {
         Entry e{};
         long long ch = 0, ud = 0;
         if (line[0] == '#') continue;
+        double fb = 0;
+        if (std::sscanf(line, "batch %lf", &fb) == 1) {
+            if (fb >= 0.05 && fb <= 20) g_batch_factor = fb;
+            continue;
+        }
         const int k = std::sscanf(line, "%d %lf %lf %lld %lld", &e.decade, &e.fx, &e.cx, &ch, &ud);
         if (k >= 4 && e.fx > 0.05 && e.fx < 100 && e.cx > 0.1 && e.cx < 5.0 && ch >= 0) {
             e.chunk = ch;
@@ -103,6 +110,21 @@ TuneChoice tune_lookup(i64 X, bool streaming) {
     t.cx = cx;
     t.ud = std::min(u, (ud / 192) * 192);
     return t;
+}
+
+// Shared cutoff of a batch.  The table is built once per call, so its cost
+// (linear in the cutoff) is paid once while the quotient work of every X
+// (proportional to X / sqrt(u)) falls as the cutoff grows: the optimum of
+// a U + sum_i b X_i / sqrt(U) is the single-X optimum at X = sum_i X_i.
+// The tuned batch factor scales it; the result is capped like tune_lookup.
+i64 tune_batch_u(double xsum) {
+    std::call_once(g_once, load);
+    const double xe = std::min(xsum, 9.0e18);
+    const TuneChoice t = tune_lookup((i64)xe, true);
+    double u = (double)t.u * g_batch_factor;
+    u = std::min(u, 200000000000.0);
+    const i64 ui = ((i64)u / 192) * 192;
+    return ui > 0 ? ui : 192;
 }
 
 i64 tune_chunks(i64 X) {

@@ -16,6 +16,7 @@ struct TuneChoice {
 TuneChoice tune_lookup(i64 X, bool streaming);
 i64 tune_chunk();
 i64 tune_chunks(i64 X);   // tuned quotient-phase chunk count for X (0 = automatic)
+i64 tune_batch_u(double xsum);   // shared cutoff of a batch whose X values sum to xsum
 
 }  // namespace mdr
 

@@ -1,0 +1,85 @@
+"""Batched and table-sharing plans on the GPU (tuned parameters).
+
+A tuned batch builds one table whose cutoff is chosen for the whole call
+(every X above it is summed with u = that cutoff, every X below it is read
+from the table), and a tuned call whose cached table already covers it reuses
+that table with the cached cutoff.  M(X) does not depend on u, so every case
+is checked bit-exactly against the compiled reference's values; the extreme
+plans this creates (u just below X, so N = X / u is 1, 2, 3 ...) are checked
+directly as well.
+"""
+import random
+
+import pytest
+
+import mdr_hip
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def tuned(**kw):
+    return mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, **kw)
+
+
+def test_tuned_batch_of_all_reference_values(engine, ref_values):
+    xs = sorted(ref_values)
+    out = engine.batch(xs, tuned(no_table_reuse=1))
+    assert out == [ref_values[x] for x in xs]
+    st = engine.last_stats
+    # the shared cutoff is far above the single-X cutoff of the largest X
+    single = engine.plan(max(xs), tuned())
+    assert st.U > single["u"]
+
+
+def test_tuned_batch_sample_of_the_4096_workload(engine, ref_values, xlists):
+    xs = xlists["batch_4096_1e9_1e14_seed3"]
+    sub = random.Random(5).sample(xs, 600)
+    for graph in (0, 1):
+        out = engine.batch(sub, tuned(no_table_reuse=1, use_graph=graph,
+                                      partition=mdr_hip.PARTITION_LEARNED))
+        want = [ref_values[x] if x in ref_values else None for x in sub]
+        assert all(w is None or o == w for o, w in zip(out, want))
+        # X without a reference value: the same X alone with reference parameters
+        rest = [i for i, w in enumerate(want) if w is None][:12]
+        for i in rest:
+            assert out[i] == engine.mertens(sub[i], no_table_reuse=1), sub[i]
+
+
+@pytest.mark.parametrize("X", [10**9 + 7, 3 * 10**10 + 1, 999999999989, 10**12])
+def test_cutoff_just_below_x(engine, X):
+    want = oracle.mertens(X, 8) if X <= 10**10 else None
+    base = engine.mertens(X, no_table_reuse=1)
+    if want is not None:
+        assert base == want
+    for k in (1, 2, 3, 4, 5, 6, 7, 11, 13, 36, 97):
+        u = X // k
+        if u > 2 * 10**11 or u * u < 9 * X:
+            continue
+        r = engine.mertens(X, tuned(tune_u=u, no_table_reuse=1))
+        assert r == base, (X, k, engine.last_stats.N)
+
+
+def test_covering_table_reuse(engine, known, ref_values):
+    # build a large table, then evaluate smaller X against it
+    assert engine.mertens(10**16, tuned()) == known[10**16]
+    big_u = engine.last_stats.U
+    xs = sorted(x for x in ref_values if 10**9 <= x <= 10**15)
+    for x in random.Random(7).sample(xs, 40):
+        assert engine.mertens(x, tuned()) == ref_values[x], x
+        st = engine.last_stats
+        assert st.table_reused == 1 and st.U == big_u
+    # reference parameters never take the covering path
+    x = xs[0]
+    assert engine.mertens(x) == ref_values[x]
+    assert engine.last_stats.U != big_u
+
+
+def test_lookup_only_batches(engine, ref_values):
+    # a batch whose X all fall below a cached cutoff is answered by lookups
+    assert engine.mertens(10**15, tuned()) is not None
+    U = engine.last_stats.U
+    xs = sorted(x for x in ref_values if 10**6 <= x <= U)[:300]
+    out = engine.batch(xs, tuned())
+    assert out == [ref_values[x] for x in xs]
+    assert engine.last_stats.s1_terms == 0 and engine.last_stats.s2_terms == 0
