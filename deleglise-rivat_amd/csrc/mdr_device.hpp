@@ -19,6 +19,19 @@ typedef unsigned __int128 u128;
 // 2^-14 seed), and the integer remainder then fixes the estimate exactly.
 // The estimate is never trusted on its own, for any y.
 // ---------------------------------------------------------------------------
+// One Newton step on v_rcp_f64: the hardware estimate is within 2^-24.3
+// relative (measured over every n < 2^31 and samples to 2^52), one step
+// squares that to <= 2^-48.6 (plus rounding).  Enough for every quotient
+// width below: the estimate y*r is within 2^-4 of y/n for q < 2^44 and within
+// 2^15 for q < 2^63, and each path ends with an exact integer correction that
+// tolerates an estimate off by one (qdiv_big: a second, exact-remainder step).
+__device__ __forceinline__ double rcp_fast(double d) {
+    const double r = __builtin_amdgcn_rcp(d);
+    const double e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// Two Newton steps (set-up code outside the hot loops).
 __device__ __forceinline__ double rcp_refined(double d) {
     double r = __builtin_amdgcn_rcp(d);
     double e = __builtin_fma(-d, r, 1.0);
@@ -46,8 +59,8 @@ __device__ __forceinline__ u64 qdiv_mid(u64 y, double dy, u64 n, double rn) {
 }
 
 // q = floor(y/n) for any y < 2^63, 1 <= n < 2^32: the first estimate is
-// within 2^13 of q, the remainder is then exact in double, a second estimate
-// fixes all but a final +-1.
+// within 2^15 of q (2^13 with two Newton steps), the remainder is then exact
+// in double, a second estimate fixes all but a final +-1.
 __device__ __forceinline__ u64 qdiv_big(u64 y, double dy, u64 n, double rn) {
     u64 q = (u64)(dy * rn);
     i64 r = (i64)(y - q * n);

@@ -140,7 +140,7 @@ __device__ i64 s1_dense(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane)
         int32_t s = 0;
         for (uint32_t i = lane; i < (uint32_t)cnt; i += 64) {
             const uint32_t n = nb + (uint32_t)ST * i;
-            const uint32_t q = qdiv32(ylo, dy, n, rcp_refined((double)n));
+            const uint32_t q = qdiv32(ylo, dy, n, rcp_fast((double)n));
             s += tp.Md[q];
         }
         return s;
@@ -148,7 +148,7 @@ __device__ i64 s1_dense(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane)
     i64 s = 0;
     for (u64 i = lane; i < cnt; i += 64) {
         const u64 n = nlo + (u64)ST * i;
-        s += tp.Md[qdiv_mid(y, dy, n, rcp_refined((double)n))];
+        s += tp.Md[qdiv_mid(y, dy, n, rcp_fast((double)n))];
     }
     return s;
 }
@@ -168,7 +168,7 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
         const u64 base = (u64)tp.ud + 1;
         for (uint32_t i = lane; i < (uint32_t)cnt; i += 64) {
             const uint32_t n = nb + (uint32_t)ST * i;
-            const double qd = dy * rcp_refined((double)n);
+            const double qd = dy * rcp_fast((double)n);
             const u64 q0 = (u64)__double_as_longlong(qd + 4503599627370496.0) & 0xfffffffffffffull;
             const uint32_t r = ylo - (uint32_t)q0 * n;
             const u64 q = q0 - (r >= 2u * n ? 1ull : 0ull);
@@ -178,7 +178,7 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
     }
     for (u64 i = lane; i < cnt; i += 64) {
         const u64 n = nlo + (u64)ST * i;
-        const u64 q = qdiv_mid(y, dy, n, rcp_refined((double)n));
+        const u64 q = qdiv_mid(y, dy, n, rcp_fast((double)n));
         s += comp_M(tp.cl, q - (u64)tp.ud - 1);
     }
     return s;
@@ -219,7 +219,7 @@ __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane)
     for (u64 t = t_lo + lane; t < ts; t += 64) {
         const u64 j = (u64)j_of_t((i64)t);
         const int code = mu6_code(tp.mu6, (i64)t);
-        const u64 q = qdiv_big(y, dy, j, rcp_refined((double)j));
+        const u64 q = qdiv_big(y, dy, j, rcp_fast((double)j));
         const i64 v = piece_eval<P, u64>(q);
         s += code == 2 ? v : (code == 0 ? -v : 0);
     }
@@ -235,7 +235,7 @@ __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane)
         const uint32_t t = tb + i;
         const uint32_t j = 3u * t + 1u + (t & 1u);
         const uint32_t code = (tp.mu6[t >> 4] >> (2 * (t & 15))) & 3u;
-        const uint32_t q = qdiv32(ylo, dy, j, rcp_refined((double)j));
+        const uint32_t q = qdiv32(ylo, dy, j, rcp_fast((double)j));
         const uint32_t v = piece32<P>(q);
         if constexpr (P == 4) {
             acc_s += (i64)((int)code - 1) * (i64)(int32_t)v;

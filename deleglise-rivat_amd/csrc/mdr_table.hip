@@ -77,57 +77,68 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
     __syncthreads();
 
-    // 2. square flags for 7 <= p < 256, block-cooperative over the multiples
-    for (int i = 0; i < a.n_sq_small; ++i) {
+    // 2. square flags for 7 <= p < 256: wave w takes primes i = w (mod 8), its
+    //    lanes the consecutive multiples (a block-wide loop per prime costs
+    //    more in per-thread set-up than the few hits it has).
+    constexpr int NWAVES = SIEVE_THREADS / 64;
+    for (int i = wave; i < a.n_sq_small; i += NWAVES) {
         const uint32_t p = a.sqprimes[i];
         const uint32_t p2 = p * p;
         if ((u64)p2 > lastn) break;
-        for (uint32_t x = sq_off[i] + (uint32_t)tid * p2; x < (uint32_t)len; x += SIEVE_THREADS * p2)
+        for (uint32_t x = sq_off[i] + (uint32_t)lane * p2; x < (uint32_t)len; x += 64u * p2)
             atomicOr(&acc32[x >> 2], 0x80u << (8 * (x & 3)));
     }
-    // 3. weights for 13 <= p < 256, block-cooperative over the multiples
-    for (int i = 0; i < a.n_small; ++i) {
-        const uint32_t p = a.wprimes[i];
-        const uint32_t w = a.wweights[i];
-        for (uint32_t x = small_off[i] + (uint32_t)tid * p; x < (uint32_t)len; x += SIEVE_THREADS * p)
+    // 3. weights for 13 <= p < 256: the block-wide loop for p < 64 (many hits
+    //    per thread), the wave loop above it
+    int i3 = 0;
+    for (; i3 < a.n_small; ++i3) {
+        const uint32_t p = a.wprimes[i3];
+        if (p >= 64) break;
+        const uint32_t w = a.wweights[i3];
+        for (uint32_t x = small_off[i3] + (uint32_t)tid * p; x < (uint32_t)len; x += SIEVE_THREADS * p)
             atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
     }
-    // 4. primes 256 <= p <= sqrt(last n), one per thread, four in flight:
+    for (int i = i3 + wave; i < a.n_small; i += NWAVES) {
+        const uint32_t p = a.wprimes[i];
+        const uint32_t w = a.wweights[i];
+        for (uint32_t x = small_off[i] + (uint32_t)lane * p; x < (uint32_t)len; x += 64u * p)
+            atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
+    }
+    // 4. primes 256 <= p <= sqrt(last n), one per thread, four in flight.
     //    r2 = first mod p^2 gives the (at most one) square hit, r2 mod p the
-    //    first weight hit.  Primes ascend, so a thread stops at the first
-    //    p with p^2 beyond the segment.
+    //    first weight hit; both in FP64, exact: first < 2^53, the quotient
+    //    estimates are within 1, and fma(-q, m, x) is the exact remainder
+    //    (|x - q m| < 2m is representable).  Primes ascend, so a thread stops
+    //    at the first p with p^2 beyond the segment.
     constexpr int UNR = 4;
+    const double dfirst = (double)first;
     for (int i0 = a.n_small + tid; i0 < a.n_w; i0 += UNR * SIEVE_THREADS) {
         uint32_t pp[UNR];
-        double rr[UNR];
-        uint32_t ww[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const int i = i0 + u * SIEVE_THREADS;
-            const bool ok = i < a.n_w;
-            pp[u] = ok ? a.wprimes[i] : 0xffffffffu;
-            rr[u] = ok ? a.wrcp[i] : 0.0;
-            ww[u] = ok ? a.wweights[i] : 0u;
+            pp[u] = i < a.n_w ? a.wprimes[i] : 0xffffffffu;
         }
         if ((u64)pp[0] * pp[0] > lastn) break;
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const u64 p = pp[u];
-            const u64 p2 = p * p;
-            if (p2 > lastn) continue;
-            const double rp = rr[u];
-            i64 q2 = (i64)((double)first * (rp * rp));
-            i64 r2 = (i64)first - q2 * (i64)p2;
-            while (r2 < 0) r2 += (i64)p2;
-            while (r2 >= (i64)p2) r2 -= (i64)p2;
-            const u64 xs = r2 ? p2 - (u64)r2 : 0;
+            const uint32_t p = pp[u];
+            if ((u64)p * p > lastn) continue;
+            const double dp = (double)p, dp2 = dp * dp;
+            const double rp = rcp_fast(dp);
+            const double q2 = __builtin_floor(dfirst * (rp * rp));
+            double r2 = __builtin_fma(-q2, dp2, dfirst);
+            if (r2 < 0) r2 += dp2;
+            else if (r2 >= dp2) r2 -= dp2;
+            const u64 xs = r2 != 0.0 ? (u64)(dp2 - r2) : 0;
             if (xs < (u64)len) atomicOr(&acc32[xs >> 2], 0x80u << (8 * (xs & 3)));
-            const i64 q1 = (i64)((double)r2 * rp);
-            i64 r = r2 - q1 * (i64)p;
-            if (r < 0) r += (i64)p;
-            else if (r >= (i64)p) r -= (i64)p;
-            const uint32_t w = ww[u];
-            for (uint32_t x = r ? (uint32_t)(p - r) : 0; x < (uint32_t)len; x += (uint32_t)p)
+            const double q1 = __builtin_floor(r2 * rp);
+            double r = __builtin_fma(-q1, dp, r2);
+            if (r < 0) r += dp;
+            else if (r >= dp) r -= dp;
+            const uint32_t ri = (uint32_t)r;
+            const uint32_t w = (32u - (uint32_t)__clz(p - 1u)) | 1u;
+            for (uint32_t x = ri ? p - ri : 0; x < (uint32_t)len; x += p)
                 atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
         }
     }

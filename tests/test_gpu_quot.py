@@ -96,3 +96,42 @@ def test_dense_only_table_g_eval(engine, mu_ref, pre_ref):
     for y, u, cx in [(10**9, 200000, 0.95), (10**11, LIM, 1.35), (987654321987, LIM, 1.15)]:
         for comb in (0, 1):
             assert engine.g_eval(y, comb, u, cx) == (oracle.g_comb2 if comb else oracle.g_single)(y, cx, u, M16, mu_ref)
+
+
+def lcm_upto(k):
+    v = 1
+    for i in range(2, k + 1):
+        v = v * i // math.gcd(v, i)
+    return v
+
+
+# y with very many divisors (remainder 0 at most n), and y -+ 1 beside them
+# (remainders n - 1 and 1): the points where an estimated quotient is off by one
+DIV_Y = []
+for base in [lcm_upto(20), lcm_upto(30), lcm_upto(40), lcm_upto(42), lcm_upto(42) * 42, 720720 * 2**40]:
+    DIV_Y += [base - 1, base, base + 1]
+
+
+@pytest.mark.parametrize("piece", [1, 4, 7, 9, 12])
+def test_s2_interval_exact_division_edges(table, mu_ref, piece):
+    for y in DIV_Y:
+        assert y < 2**63
+        for first, last in [(1, 4000), (1, LIM), (LIM // 2, LIM)]:
+            assert table.s2_interval(piece, first, last, y) == oracle.s2_interval(piece, first, last, y, mu_ref), \
+                (piece, y, first, last)
+
+
+def test_s1_sums_exact_division_edges(table, pre_ref):
+    # S1 reads M(y/n) <= LIM from this table, so y/LIM <= n <= sqrt(y) needs y <= LIM^2
+    ran = 0
+    for y in DIV_Y:
+        first = y // LIM + 1
+        last = min(first + 200000, math.isqrt(y))
+        if last < first:
+            continue
+        for stride in (1, 2):
+            n = np.arange(first, last + 1, stride, dtype=object)
+            exp = int(sum(int(pre_ref[y // int(v)]) for v in n))
+            assert table.s1_sum(y, first, last, stride) == exp, (y, first, last, stride)
+            ran += 1
+    assert ran >= 12
