@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-static", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the 64-X batch in [1e16, 1e18]")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path would shard by "
@@ -189,6 +190,29 @@ def main():
         log("batch", batch)
         assert bad == 0
 
+    large = None
+    if not args.no_large:
+        # 64 X log-uniform in [1e16, 1e18], seed 5: one call sharing one table
+        xs = load_json("xlists.json")["large_64_1e16_1e18_seed5"]
+        lo = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
+                               sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
+        t0 = time.perf_counter()
+        out = E.batch(xs, lo)
+        tl = time.perf_counter() - t0
+        lst = E.last_stats
+        checked = [i for i, x in enumerate(xs) if x in ref]
+        bad = sum(out[i] != ref[xs[i]] for i in checked)
+        # invariance: the median X alone, with its own table and cutoff
+        mid = sorted(range(len(xs)), key=lambda i: xs[i])[len(xs) // 2]
+        alone = E.mertens(xs[mid], mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1))
+        bad += int(alone != out[mid])
+        large = {"workload": "64 X log-uniform in [1e16, 1e18], seed 5, one batch", "n": len(xs),
+                 "seconds": round(tl, 3), "evals_per_s": round(len(xs) / tl, 2), "table_U": lst.U,
+                 "table_ms": round(lst.table_ms, 1), "quotient_ms": round(lst.quotient_ms, 1),
+                 "checked_vs_reference": len(checked), "checked_alone": 1, "mismatches": bad}
+        log("large batch", large)
+        assert bad == 0
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -219,6 +243,7 @@ def main():
                            "frac": round(st.table_bytes / (t_table / 1000) / 1e9 / HBM_PEAK_GBS, 4),
                            "bytes": st.table_bytes, "ms": round(t_table, 3)},
         "batch": batch,
+        "large_batch": large,
         "cpu_baseline": cpu,
     }
     E.close()

@@ -31,6 +31,8 @@ constexpr int STENCIL_PERIOD = 277200;
 // Values per compressed line; u_dense must be a multiple of it below U, and
 // SEG is a multiple of it, so lines never straddle segments.
 constexpr int LINE_VALUES = 192;
+// Largest compressed tier (U - u_dense): line indices stay below 2^32.
+constexpr long long MAX_COMPRESSED_VALUES = (long long)LINE_VALUES * 4294967295LL;
 
 struct alignas(64) CLine {
     u64 pl[6];        // word w: pl[2w] = nonzero plane, pl[2w+1] = negative plane

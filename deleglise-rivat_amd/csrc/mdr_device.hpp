@@ -116,15 +116,19 @@ __device__ __forceinline__ int mu6_code(const uint32_t* mu6, i64 t) {
     return (int)((mu6[t >> 4] >> (2 * (t & 15))) & 3u);
 }
 
-// M(q) in the compressed tier (q > ud): the value's line, word and bit.
-__device__ __forceinline__ int32_t comp_M(const CLine* cl, u64 off) {
-    const uint32_t w = (uint32_t)(off >> 6);     // off < 2^38
-    const uint32_t line = w / 3u;
-    const uint32_t sub = w - 3u * line;
-    const CLine* L = cl + line;
+// M from one compressed line: word sub (0..2), bit b (0..63) of the word.
+__device__ __forceinline__ int32_t line_M(const CLine* L, uint32_t sub, uint32_t b) {
     const u64 nz = L->pl[2 * sub], ng = L->pl[2 * sub + 1];
-    const u64 mask = ~0ull >> (63 - (off & 63));
+    const u64 mask = ~0ull >> (63 - b);
     return L->ck[sub] + (int)__popcll(nz & mask) - 2 * (int)__popcll(ng & mask);
+}
+
+// M(q) in the compressed tier (q > ud), off = q - ud - 1 < 192 * 2^32: the
+// value's line off / 192, word (off >> 6) mod 3 and bit off & 63.
+__device__ __forceinline__ int32_t comp_M(const CLine* cl, u64 off) {
+    const u64 line = off / (u64)LINE_VALUES;
+    const uint32_t sub = (uint32_t)(off >> 6) - 3u * (uint32_t)line;
+    return line_M(cl + line, sub, (uint32_t)(off & 63));
 }
 
 // M(q) from either tier (mertens_dr.hpp:707-712, :741-752).

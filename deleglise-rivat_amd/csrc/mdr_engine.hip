@@ -189,6 +189,8 @@ mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
         return fail(c, MDR_ERR_INVALID, "tuning override out of range");
     if (o.tune_u > 1000000000000LL) return fail(c, MDR_ERR_INVALID, "tune_u above 1e12");
     if (o.use_graph < 0 || o.use_graph > 1) return fail(c, MDR_ERR_INVALID, "use_graph must be 0 or 1");
+    if (o.tune_batch != 0 && !(o.tune_batch >= 0.05 && o.tune_batch <= 20))
+        return fail(c, MDR_ERR_INVALID, "tune_batch outside [0.05, 20]");
     return MDR_OK;
 }
 
@@ -256,7 +258,7 @@ void share_batch_cutoff(std::vector<XPlan>& plans, const mdr_opts& o) {
             umax = std::max(umax, p.u);
         }
     if (nsum < 2) return;
-    const i64 ub = tune_batch_u(xsum);
+    const i64 ub = tune_batch_u(xsum, o.tune_batch);
     if (ub <= umax) return;
     for (XPlan& p : plans) {
         if (p.kind != 2 || p.u >= ub) continue;
@@ -342,6 +344,8 @@ mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, Tabl
         return fail(c, MDR_ERR_INVALID, "u_dense must be a multiple of 192 below U");
     if (mu_lim > U) return fail(c, MDR_ERR_INVALID, "mu_limit above U");
     if (mu_lim < 1) mu_lim = 1;
+    if (U - ud > MAX_COMPRESSED_VALUES)
+        return fail(c, MDR_ERR_RANGE, "table above the compressed-tier limit (U - u_dense > 192 * (2^32 - 1))");
     j = TableJob{};
     j.U = U;
     j.ud = ud;
@@ -639,10 +643,15 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
         b.learned = o.partition == MDR_PARTITION_LEARNED && c->mlp_ok;
         const i64 nwaves = (i64)c->ncu * c->quot_blocks_per_cu * 4;
         i64 tuned_chunks = 0;
-        if (o.params == MDR_PARAMS_TUNED) tuned_chunks = tune_chunks(plans.back().X);
+        if (o.params == MDR_PARAMS_TUNED) {
+            i64 xmax = 1;
+            for (const XPlan& p : plans)
+                if (p.kind == 2) xmax = std::max(xmax, p.X);
+            tuned_chunks = tune_chunks(xmax);
+        }
         b.P = o.chunk > 0 ? o.chunk
               : tuned_chunks > 0 ? tuned_chunks
-                                 : nwaves * (o.sched == MDR_SCHED_DYNAMIC ? 16 : 4);
+                                 : nwaves * (o.sched == MDR_SCHED_DYNAMIC ? 64 : 4);
         HIPT(c->xparam.ensure(b.nxs * sizeof(XParam)), "alloc params");
         HIPT(c->slot_len.ensure((size_t)slots * 8), "alloc slot lengths");
         if (b.learned) HIPT(c->slot_cost.ensure((size_t)slots * 8), "alloc slot costs");
@@ -667,6 +676,8 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
         pa.totals = reinterpret_cast<const i64*>(ctr + 4);
         pa.P = b.P;
         pa.bounds = c->bounds.as<i64>();
+        // automatic chunk counts adapt to the call's work (explicit ones are kept)
+        pa.min_chunk = o.chunk > 0 ? 0 : MIN_CHUNK_POSITIONS;
         QuotArgs& qa = b.qa;
         qa.xp = pa.xp;
         qa.nx = pa.nx;
@@ -801,6 +812,8 @@ mdr_status mdr_init(mdr_ctx** out, int device) {
         return MDR_ERR_HIP;
     }
     c->quot_blocks_per_cu = std::max(1, occ);
+    size_t mem_free = 0, mem_total = 0;
+    if (hipMemGetInfo(&mem_free, &mem_total) == hipSuccess && mem_total > 0) tune_set_device_memory(mem_total);
     // the trained cost model: MDR_COST_MODEL, else cost_model.bin beside the library
     std::string path;
     if (const char* e = std::getenv("MDR_COST_MODEL")) path = e;

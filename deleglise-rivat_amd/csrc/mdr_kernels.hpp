@@ -34,6 +34,9 @@ struct FixupArgs {
 // Fixed per-slot cost (cycles) charged by the learned partition for a G
 // evaluation's set-up (frame, segment geometry, constant).
 constexpr long long SLOT_OVERHEAD_CYCLES = 1500;
+// Smallest automatic chunk (quotient positions): a chunk's set-up (locating its
+// slots, building their frames) costs about as much as a few thousand positions.
+constexpr long long MIN_CHUNK_POSITIONS = 1LL << 15;
 
 // Partition of the quotient phase.
 struct PartArgs {
@@ -46,8 +49,10 @@ struct PartArgs {
     i64* cost;                     // per slot predicted cycles; cost offsets after the scan
     unsigned long long* type_pos;  // [3]: S1 dense, S1 compressed, S2 positions
     const i64* totals;             // [0] total positions, [1] total predicted cycles
-    i64 P;                         // chunks
+    i64 P;                         // chunks (bounds has P + 1 entries)
     i64* bounds;                   // [P + 1]
+    i64 min_chunk;                 // > 0: use only max(1, min(P, T / min_chunk)) chunks,
+                                   // the rest stay empty (small calls, little set-up)
 };
 
 struct QuotArgs {

@@ -81,18 +81,24 @@ __global__ void k_slot_len(PartArgs a) {
     }
 }
 
-// Chunk boundaries B[0..P]: static c*T/P, or the position where the running
-// predicted cost reaches c*C/P (monotone in c, B[0] = 0, B[P] = T).
+// Chunk boundaries B[0..P]: with Pe = P (or fewer, see min_chunk) chunks in
+// use, static c*T/Pe, or the position where the running predicted cost
+// reaches c*C/Pe; B[c] = T for c >= Pe (monotone in c, B[0] = 0, B[P] = T).
 __global__ void k_cuts(PartArgs a) {
     const i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (c > a.P) return;
     const i64 T = a.totals[0];
-    if (!a.mlp || c == 0 || c == a.P) {
-        a.bounds[c] = (i64)((u128)(u64)c * (u64)T / (u64)a.P);
+    i64 Pe = a.P;
+    if (a.min_chunk > 0) {
+        const i64 want = T / a.min_chunk;
+        Pe = want < 1 ? 1 : (want < a.P ? want : a.P);
+    }
+    if (!a.mlp || c == 0 || c >= Pe) {
+        a.bounds[c] = c >= Pe ? T : (i64)((u128)(u64)c * (u64)T / (u64)Pe);
         return;
     }
     const i64 Ctot = a.totals[1];
-    const i64 target = (i64)((u128)(u64)c * (u64)Ctot / (u64)a.P);
+    const i64 target = (i64)((u128)(u64)c * (u64)Ctot / (u64)Pe);
     const i64 G = last_le(a.cost, a.total_slots, target);
     const i64 base = a.len[G];
     const i64 Lg = (G + 1 < a.total_slots ? a.len[G + 1] : T) - base;
@@ -162,17 +168,25 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
         // n < 2^32 with 3n <= 2^32, q <= U < 2^44.  q0 = nearest integer to
         // the estimate (2^52 + qd leaves it in the low mantissa bits); q0 is q
         // or q + 1, so the true remainder y - q0*n lies in [-n, n) and its low
-        // 32 bits decide the correction exactly.
+        // 32 bits decide the correction exactly.  The line index off / 192 is
+        // taken in FP64: off is exact there (< 2^52) and (off + 1/2) / 192 is
+        // at least 1/384 away from an integer, far above the rounding error.
         const uint32_t ylo = (uint32_t)y;
         const uint32_t nb = (uint32_t)nlo;
         const u64 base = (u64)tp.ud + 1;
+        const double cbase = 4503599627370496.0 + (double)base;   // 2^52 + base, exact
         for (uint32_t i = lane; i < (uint32_t)cnt; i += 64) {
             const uint32_t n = nb + (uint32_t)ST * i;
-            const double qd = dy * rcp_fast((double)n);
-            const u64 q0 = (u64)__double_as_longlong(qd + 4503599627370496.0) & 0xfffffffffffffull;
+            const double qm = __builtin_fma(dy, rcp_fast((double)n), 4503599627370496.0);
+            const u64 q0 = (u64)__double_as_longlong(qm) & 0xfffffffffffffull;
             const uint32_t r = ylo - (uint32_t)q0 * n;
-            const u64 q = q0 - (r >= 2u * n ? 1ull : 0ull);
-            s += comp_M(tp.cl, q - base);
+            const bool over = r >= 2u * n;                 // q0 = q + 1
+            const u64 off = q0 - base - (over ? 1ull : 0ull);
+            const double offd = (qm - cbase) - (over ? 1.0 : 0.0);
+            const uint32_t line = (uint32_t)__builtin_floor(
+                __builtin_fma(offd, 1.0 / LINE_VALUES, 0.5 / LINE_VALUES));
+            const uint32_t sub = (uint32_t)(off >> 6) - 3u * line;
+            s += line_M(tp.cl + line, sub, (uint32_t)(off & 63));
         }
         return s;
     }
@@ -306,6 +320,7 @@ __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
     for (i64 c = a.dynamic ? fetch_next(a.next, lane) : uniform64(wave); c < a.P;
          c = a.dynamic ? fetch_next(a.next, lane) : c + nwaves) {
         const i64 b0 = a.bounds[c], b1 = a.bounds[c + 1];
+        if (b0 >= T) break;   // bounds ascend: every later chunk is empty too
         if (b0 >= b1) continue;
         for (i64 G = last_le(a.off, a.nslots, b0); G < a.nslots; ++G) {
             const i64 og = a.off[G];

@@ -195,8 +195,9 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
         gsum[k] = (int)__popcll(nz) - 2 * (int)__popcll(ng);
         const u64 n0 = first + (u64)k * 64;
         if (n0 > (u64)a.ud && n0 <= (u64)a.U) {
-            const uint32_t w = (uint32_t)((n0 - (u64)a.ud - 1) >> 6);
-            const uint32_t line = w / 3u, sub = w - 3u * line;
+            const u64 w = (n0 - (u64)a.ud - 1) >> 6;
+            const u64 line = w / 3u;
+            const uint32_t sub = (uint32_t)(w - 3u * line);
             u64* pl = a.cl[line].pl + 2 * sub;
             pl[0] = nz;
             pl[1] = ng;
@@ -227,8 +228,9 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     for (int k = tid; k < GROUPS; k += SIEVE_THREADS) {
         const u64 n0 = first + (u64)k * 64;
         if (n0 > (u64)a.ud && n0 <= (u64)a.U) {
-            const uint32_t w = (uint32_t)((n0 - (u64)a.ud - 1) >> 6);
-            const uint32_t line = w / 3u, sub = w - 3u * line;
+            const u64 w = (n0 - (u64)a.ud - 1) >> 6;
+            const u64 line = w / 3u;
+            const uint32_t sub = (uint32_t)(w - 3u * line);
             a.cl[line].ck[sub] = gpre[k];
         }
     }

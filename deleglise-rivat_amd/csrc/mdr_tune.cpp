@@ -12,6 +12,7 @@
 #include "mdr_tune.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -35,6 +36,10 @@ struct Entry {
 std::once_flag g_once;
 std::vector<Entry> g_entries;
 double g_batch_factor = 1.0;
+// Largest tuned cutoff: 4e11 values (133 GB of compressed lines), lowered to
+// what the device's memory holds (tune_set_device_memory).
+constexpr long long U_CAP_MAX = 400000000000LL;
+std::atomic<long long> g_u_cap{U_CAP_MAX};
 
 void load() {
     std::string path;
@@ -103,7 +108,7 @@ TuneChoice tune_lookup(i64 X, bool streaming) {
     u = std::max(u, 3 * S);
     const i64 ends = (i64)(cx * std::sqrt((double)X)) + 1000;
     u = std::max(u, ends + 64);
-    u = std::min<i64>(u, 200000000000LL);
+    u = std::min<i64>(u, g_u_cap.load());
     u = ((u + 191) / 192) * 192;
     TuneChoice t;
     t.u = u;
@@ -116,13 +121,14 @@ TuneChoice tune_lookup(i64 X, bool streaming) {
 // (linear in the cutoff) is paid once while the quotient work of every X
 // (proportional to X / sqrt(u)) falls as the cutoff grows: the optimum of
 // a U + sum_i b X_i / sqrt(U) is the single-X optimum at X = sum_i X_i.
-// The tuned batch factor scales it; the result is capped like tune_lookup.
-i64 tune_batch_u(double xsum) {
+// The tuned batch factor (or factor > 0) scales it; the result is capped
+// like tune_lookup.
+i64 tune_batch_u(double xsum, double factor) {
     std::call_once(g_once, load);
     const double xe = std::min(xsum, 9.0e18);
     const TuneChoice t = tune_lookup((i64)xe, true);
-    double u = (double)t.u * g_batch_factor;
-    u = std::min(u, 200000000000.0);
+    double u = (double)t.u * (factor > 0 ? factor : g_batch_factor);
+    u = std::min(u, (double)g_u_cap.load());
     const i64 ui = ((i64)u / 192) * 192;
     return ui > 0 ? ui : 192;
 }
@@ -136,5 +142,11 @@ i64 tune_chunks(i64 X) {
 }
 
 i64 tune_chunk() { return tune_chunks(1); }
+
+void tune_set_device_memory(size_t bytes) {
+    // compressed lines take 1/3 byte per value: 45 % of the memory at most
+    const double cap = std::min((double)U_CAP_MAX, (double)bytes * 0.45 * 3.0);
+    g_u_cap.store(std::max<long long>(20000000000LL, (long long)cap));
+}
 
 }  // namespace mdr

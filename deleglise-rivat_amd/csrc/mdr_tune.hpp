@@ -3,6 +3,8 @@
 #ifndef MDR_TUNE_HPP
 #define MDR_TUNE_HPP
 
+#include <stddef.h>
+
 #include "mdr_common.hpp"
 
 namespace mdr {
@@ -16,7 +18,10 @@ struct TuneChoice {
 TuneChoice tune_lookup(i64 X, bool streaming);
 i64 tune_chunk();
 i64 tune_chunks(i64 X);   // tuned quotient-phase chunk count for X (0 = automatic)
-i64 tune_batch_u(double xsum);   // shared cutoff of a batch whose X values sum to xsum
+// shared cutoff of a batch whose X values sum to xsum (factor 0 = tuned factor)
+i64 tune_batch_u(double xsum, double factor = 0);
+// Lowers the largest tuned cutoff to what a device of this memory holds.
+void tune_set_device_memory(size_t bytes);
 
 }  // namespace mdr
 

@@ -73,12 +73,16 @@ typedef struct {
     int record_cycles;        /* s > 0: log cycles of every cost piece of 1 in s G evaluations */
     int no_table_reuse;       /* 1: rebuild the table even when the cached one fits */
     /* explicit GPU split for MDR_PARAMS_TUNED (0 = from the tuning cache):
-     * sieve cutoff, dense-tier limit and S1/S2 split factor */
+     * sieve cutoff (<= 1e12; a table whose compressed tier U - u_dense
+     * exceeds 192 * (2^32 - 1) values fails with MDR_ERR_RANGE), dense-tier
+     * limit and S1/S2 split factor */
     long long tune_u;
     long long tune_ud;
     double tune_cx;
     int use_graph;            /* 1: capture the call's launches into a hipGraph and replay it
                                  (instantiated once per launch sequence, updated in place) */
+    double tune_batch;        /* batch cutoff factor for MDR_PARAMS_TUNED batches
+                                 (0 = from the tuning cache; accepted 0.05..20) */
 } mdr_opts;
 
 typedef struct {
@@ -139,8 +143,9 @@ mdr_status mdr_plan(mdr_ctx* ctx, long long X, const mdr_opts* opts, int* stream
                     long long* u_dense, long long* mu_limit, long long* N, double* cx);
 
 /* Component entry points (parity tests).  mdr_table_build sieves n = 1..U
- * with the dense tier up to u_dense (a multiple of 192 when below U) and packed
- * mu up to mu_limit; the reads copy M(first..first+count-1) and mu(...) to host. */
+ * with the dense tier up to u_dense (a multiple of 192 when below U; U - u_dense
+ * at most 192 * (2^32 - 1), else MDR_ERR_RANGE) and packed mu up to mu_limit; the reads
+ * copy M(first..first+count-1) and mu(...) to host. */
 mdr_status mdr_table_build(mdr_ctx* ctx, long long U, long long u_dense, long long mu_limit);
 mdr_status mdr_table_read_M(mdr_ctx* ctx, long long first, long long count, int32_t* out);
 mdr_status mdr_table_read_mu(mdr_ctx* ctx, long long first, long long count, int8_t* out);

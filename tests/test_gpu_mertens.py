@@ -108,3 +108,22 @@ def test_dropin_cpp_program():
     r = subprocess.run([exe, str(10**16)], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_largest_table_and_external_1e18():
+    """M(1e18) = -46758740 is not a reference value (OEIS A084237): a
+    cross-check at the largest X of the batch workloads, with the tuned cutoff
+    and with a 3.9e11-value table (U - u_dense above 2^38, so word indices
+    above 2^32 and line indices near 2^31 are read)."""
+    e = mdr_hip.Engine(0)
+    try:
+        X = 10**18
+        assert e.mertens(X, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1)) == -46758740
+        ud = 1048512
+        U = 390_000_000_000 // 192 * 192
+        assert U - ud > 2**38
+        o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=U, tune_ud=ud, no_table_reuse=1)
+        assert e.mertens(X, o) == -46758740
+        assert e.last_stats.U == U
+    finally:
+        e.close()

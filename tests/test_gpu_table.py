@@ -9,6 +9,7 @@ import math
 import numpy as np
 import pytest
 
+import mdr_hip
 import oracle
 from test_oracle import segmented_mobius
 
@@ -81,3 +82,14 @@ def test_read_beyond_table_is_an_error(engine):
     engine.table_build(1000, 1000, 1000)
     with pytest.raises(mdr_hip.MdrError):
         engine.read_M(900, 200)
+
+
+def test_compressed_tier_limit_is_an_error(engine):
+    # the compressed tier holds at most 192 * (2^32 - 1) values (32-bit line
+    # indices); larger requests fail before anything is allocated
+    ud = 1048512
+    with pytest.raises(mdr_hip.MdrError, match="RANGE|compressed-tier"):
+        engine.table_build(ud + 192 * 2**32, ud, 10**6)
+    with pytest.raises(mdr_hip.MdrError, match="RANGE|compressed-tier"):
+        engine.mertens(10**18, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=9 * 10**11,
+                                                 tune_ud=ud, no_table_reuse=1))
