@@ -1,14 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X Deleglise-Rivat engine.
 
-Metric (BASELINE): M(X) wall time at X = 1e17, learned versus static S2
-balance; batched M(X) evaluations per second.
+Metric: M(X) wall time at X = 1e17, learned versus static S2 balance;
+batched M(X) evaluations per second.
 
 One step = one exact M(1e17) through the C ABI, table build included (the
 table is rebuilt every step; X is the only input and lives on the host as an
 integer argument, so nothing needs staging).  `value` is evaluations per
 second over the K timed steps.  Beside it:
-  - the static-partition time of the same X (learned/static ratio),
+  - the static-partition times of the same X: equal quotient positions per
+    chunk, and whole terms per chunk (the reference GPU engine's unit), under
+    the dynamic chunk queue and under a one-chunk-per-wave round robin,
   - batched evals/s for the 4096-X workload (seed 3, [1e9, 1e14]) against
     one-at-a-time calls,
   - the quotient kernel's roofline (algorithmic bytes / kernel time),
@@ -119,6 +121,9 @@ def main():
                                   sched=mdr_hip.SCHED_STATIC, chunk=nchunk, no_table_reuse=1)
     static_q = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
                                  sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
+    # the reference GPU engine's partition: whole terms (G evaluations) per chunk
+    terms_q = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_TERMS,
+                                sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
 
     def timed(opts, k, tag):
         res, stats = [], []
@@ -144,7 +149,9 @@ def main():
         t_sq, _ = timed(static_q, args.steps, "static+queue")
         t_lrr, _ = timed(learned_rr, args.steps, "learned+round-robin")
         t_srr, _ = timed(static_rr, args.steps, "static+round-robin")
+        t_tq, _ = timed(terms_q, 1, "terms+queue")
         cmp = {"schedule_round_robin_chunks": nchunk,
+               "wall_s_terms_queue": round(t_tq, 4), "terms_over_learned_queue": round(t_tq / t_learned, 3),
                "wall_s_learned_round_robin": round(t_lrr, 4), "wall_s_static_round_robin": round(t_srr, 4),
                "static_over_learned_round_robin": round(t_srr / t_lrr, 3),
                "wall_s_static_queue": round(t_sq, 4), "static_over_learned_queue": round(t_sq / t_learned, 3)}

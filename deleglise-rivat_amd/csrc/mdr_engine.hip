@@ -181,7 +181,7 @@ mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
     if (o.fx != 0 && !(o.fx >= 0.1 && o.fx <= 3.0)) return fail(c, MDR_ERR_INVALID, "fx outside [0.1, 3.0]");
     if (o.u_cap_dense != 0 && (o.u_cap_dense < 100000000LL || o.u_cap_dense > 4000000000LL))
         return fail(c, MDR_ERR_INVALID, "u_cap_dense outside [1e8, 4e9]");
-    if (o.partition < 0 || o.partition > 1) return fail(c, MDR_ERR_INVALID, "partition out of range");
+    if (o.partition < 0 || o.partition > 2) return fail(c, MDR_ERR_INVALID, "partition out of range");
     if (o.params < 0 || o.params > 1) return fail(c, MDR_ERR_INVALID, "params out of range");
     if (o.sched < 0 || o.sched > 1) return fail(c, MDR_ERR_INVALID, "sched out of range");
     if (o.chunk < 0) return fail(c, MDR_ERR_INVALID, "chunk must be >= 0");
@@ -678,6 +678,7 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
         pa.bounds = c->bounds.as<i64>();
         // automatic chunk counts adapt to the call's work (explicit ones are kept)
         pa.min_chunk = o.chunk > 0 ? 0 : MIN_CHUNK_POSITIONS;
+        pa.by_slot = o.partition == MDR_PARTITION_TERMS ? 1 : 0;
         QuotArgs& qa = b.qa;
         qa.xp = pa.xp;
         qa.nx = pa.nx;
@@ -763,7 +764,9 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
             stats->s1_terms += (long long)(ch[1] + ch[2]);
             stats->s1_comp_terms += (long long)ch[2];
             stats->s2_terms += (long long)ch[3];
-            stats->partition_used = b.learned ? MDR_PARTITION_LEARNED : MDR_PARTITION_STATIC;
+            stats->partition_used = b.learned ? MDR_PARTITION_LEARNED
+                                    : o.partition == MDR_PARTITION_TERMS ? MDR_PARTITION_TERMS
+                                                                         : MDR_PARTITION_STATIC;
         }
         stats->U = c->tU;
         stats->u_dense = c->tud;

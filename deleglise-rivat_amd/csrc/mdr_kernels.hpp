@@ -53,6 +53,7 @@ struct PartArgs {
     i64* bounds;                   // [P + 1]
     i64 min_chunk;                 // > 0: use only max(1, min(P, T / min_chunk)) chunks,
                                    // the rest stay empty (small calls, little set-up)
+    int by_slot;                   // 1: cut at slot boundaries, equal slot counts per chunk
 };
 
 struct QuotArgs {

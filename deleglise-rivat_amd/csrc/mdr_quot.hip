@@ -93,6 +93,11 @@ __global__ void k_cuts(PartArgs a) {
         const i64 want = T / a.min_chunk;
         Pe = want < 1 ? 1 : (want < a.P ? want : a.P);
     }
+    if (a.by_slot) {   // whole G evaluations: chunk c starts at slot c * slots / Pe
+        const i64 G = (i64)((u128)(u64)c * (u64)a.total_slots / (u64)Pe);
+        a.bounds[c] = (c >= Pe || G >= a.total_slots) ? T : a.len[G];
+        return;
+    }
     if (!a.mlp || c == 0 || c >= Pe) {
         a.bounds[c] = c >= Pe ? T : (i64)((u128)(u64)c * (u64)T / (u64)Pe);
         return;

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdr_hip.so")
 
 STRATEGY_AUTO, STRATEGY_DENSE, STRATEGY_STREAMING, STRATEGY_FORCE_STREAMING = 0, 1, 2, 3
-PARTITION_STATIC, PARTITION_LEARNED = 0, 1
+PARTITION_STATIC, PARTITION_LEARNED, PARTITION_TERMS = 0, 1, 2
 PARAMS_REFERENCE, PARAMS_TUNED = 0, 1
 SCHED_DYNAMIC, SCHED_STATIC = 0, 1
 

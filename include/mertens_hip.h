@@ -53,7 +53,11 @@ enum {
     MDR_STRATEGY_STREAMING = 2,  /* variable set to 1/y/Y/t/T: streaming from 1e11 */
     MDR_STRATEGY_FORCE_STREAMING = 3  /* compute_compressed: streaming for X > 1e5 */
 };
-enum { MDR_PARTITION_STATIC = 0, MDR_PARTITION_LEARNED = 1 };
+/* chunk partition of the quotient phase: static = equal quotient positions per
+ * chunk; learned = equal predicted cycles (the cost model); terms = equal
+ * numbers of outer terms (G evaluations) per chunk, the reference GPU engine's
+ * unit of work (one term per warp, mertens_cuda.cuh:307-403) */
+enum { MDR_PARTITION_STATIC = 0, MDR_PARTITION_LEARNED = 1, MDR_PARTITION_TERMS = 2 };
 enum { MDR_PARAMS_REFERENCE = 0, MDR_PARAMS_TUNED = 1 };
 enum { MDR_SCHED_DYNAMIC = 0, MDR_SCHED_STATIC = 1 };
 

@@ -74,7 +74,9 @@ def test_invariance_full_size(engine, known):
     X, v = 10**16, known[10**16]
     for kw in [dict(cx=0.6), dict(cx=2.4), dict(u_cap=2 * 10**9), dict(u_cap=10**10),
                dict(strategy=mdr_hip.STRATEGY_DENSE, u_cap_dense=4 * 10**9), dict(chunk=5000),
-               dict(chunk=1 << 20), dict(sched=mdr_hip.SCHED_STATIC, chunk=20000), dict(params=mdr_hip.PARAMS_TUNED)]:
+               dict(chunk=1 << 20), dict(sched=mdr_hip.SCHED_STATIC, chunk=20000), dict(params=mdr_hip.PARAMS_TUNED),
+               dict(partition=mdr_hip.PARTITION_TERMS), dict(partition=mdr_hip.PARTITION_TERMS, chunk=777),
+               dict(partition=mdr_hip.PARTITION_TERMS, sched=mdr_hip.SCHED_STATIC, params=mdr_hip.PARAMS_TUNED)]:
         print("invariance", kw, flush=True)
         assert engine.mertens(X, **kw) == v, kw
 
@@ -94,7 +96,10 @@ def test_batch_config_sample(engine, xlists, ref_values):
 
 def test_batch_equals_single(engine, xlists):
     xs = xlists["batch_4096_1e9_1e14_seed3"][:64]
-    assert engine.batch(xs) == [engine.mertens(x) for x in xs]
+    single = [engine.mertens(x) for x in xs]
+    assert engine.batch(xs) == single
+    assert engine.batch(xs, partition=mdr_hip.PARTITION_TERMS) == single
+    assert engine.last_stats.partition_used == mdr_hip.PARTITION_TERMS
 
 
 def test_large_batch_smallest_values(engine, xlists, ref_values):
