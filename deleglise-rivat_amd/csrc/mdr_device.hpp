@@ -31,6 +31,28 @@ __device__ __forceinline__ double rcp_fast(double d) {
     return __builtin_fma(r, e, r);
 }
 
+// Two reciprocals from one v_rcp_f64 (the transcendental costs about four
+// plain FP64 operations): 1/(a b) refined once, times b and times a.  Each
+// result is within 2^-48.3 relative (a b rounds by 2^-53, the refined
+// reciprocal is within 2^-48.6, the products round by 2^-53); a, b < 2^32.
+__device__ __forceinline__ void rcp_pair(double a, double b, double& ra, double& rb) {
+    const double r = rcp_fast(a * b);
+    ra = r * b;
+    rb = r * a;
+}
+
+// q = floor(y/n) for q <= 2^32 - 2 and 2n <= 2^32, from an estimate rn of 1/n
+// within 2^-47 relative.  2^52 + y*rn (one FMA) rounds the estimate, which is
+// within 2^-15 of y/n, to the nearest integer q0 in {q, q + 1} and leaves q0 in
+// the low mantissa word; the 32-bit remainder y - q0*n is in [0, n) for
+// q0 = q and in [2^32 - n, 2^32) for q0 = q + 1, so one comparison corrects.
+__device__ __forceinline__ uint32_t qdiv32n(uint32_t y_lo, double dy, uint32_t n, double rn) {
+    const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
+    const uint32_t q0 = (uint32_t)__double_as_longlong(qm);
+    const uint32_t r = y_lo - q0 * n;
+    return q0 - (r >= n ? 1u : 0u);
+}
+
 // Two Newton steps (set-up code outside the hot loops).
 __device__ __forceinline__ double rcp_refined(double d) {
     double r = __builtin_amdgcn_rcp(d);

@@ -141,18 +141,40 @@ __global__ void k_cuts(PartArgs a) {
 // Chunk evaluators.  Lanes take consecutive positions (stride 64), so table
 // gathers of neighbouring n coalesce wherever y/n^2 is small.
 // ---------------------------------------------------------------------------
+// Per-lane iteration count of a run of cnt positions taken with stride 64.
+__device__ __forceinline__ uint32_t lane_iters(u64 cnt, int lane) {
+    return (u64)lane < cnt ? (uint32_t)((cnt - (u64)lane + 63) >> 6) : 0u;
+}
+
 template <int ST>
 __device__ i64 s1_dense(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) {
     const double dy = (double)y;
     const u64 nmax = nlo + (u64)ST * (cnt - 1);
-    if (nmax < 1431655765ull) {   // 3n <= 2^32: 32-bit remainder test
+    if (nmax < 2147483648ull) {   // 2n <= 2^32: qdiv32n
+        // lane positions n, n + 64 ST, ...: two per step share one reciprocal.
+        // int32 partial sums of at most 2^15 steps (|M| < 2^15 in the int16
+        // tier) are widened before they can overflow.
         const uint32_t ylo = (uint32_t)y;
-        const uint32_t nb = (uint32_t)nlo;
-        int32_t s = 0;
-        for (uint32_t i = lane; i < (uint32_t)cnt; i += 64) {
-            const uint32_t n = nb + (uint32_t)ST * i;
-            const uint32_t q = qdiv32(ylo, dy, n, rcp_fast((double)n));
-            s += tp.Md[q];
+        const uint32_t iters = lane_iters(cnt, lane);
+        uint32_t n = (uint32_t)nlo + (uint32_t)(ST * lane);
+        constexpr uint32_t D = 64u * ST;
+        i64 s = 0;
+        for (uint32_t k0 = 0; k0 < iters; k0 += 32768u) {
+            const uint32_t kend = iters - k0 < 32768u ? iters : k0 + 32768u;
+            int32_t part = 0;
+            uint32_t k = k0;
+            for (; k + 2 <= kend; k += 2) {
+                double r1, r2;
+                rcp_pair((double)n, (double)(n + D), r1, r2);
+                part += tp.Md[qdiv32n(ylo, dy, n, r1)];
+                part += tp.Md[qdiv32n(ylo, dy, n + D, r2)];
+                n += 2 * D;
+            }
+            if (k < kend) {
+                part += tp.Md[qdiv32n(ylo, dy, n, rcp_fast((double)n))];
+                n += D;
+            }
+            s += part;
         }
         return s;
     }
@@ -169,30 +191,41 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
     const double dy = (double)y;
     const u64 nmax = nlo + (u64)ST * (cnt - 1);
     i64 s = 0;
-    if (nmax < 1431655765ull) {
-        // n < 2^32 with 3n <= 2^32, q <= U < 2^44.  q0 = nearest integer to
-        // the estimate (2^52 + qd leaves it in the low mantissa bits); q0 is q
-        // or q + 1, so the true remainder y - q0*n lies in [-n, n) and its low
-        // 32 bits decide the correction exactly.  The line index off / 192 is
-        // taken in FP64: off is exact there (< 2^52) and (off + 1/2) / 192 is
-        // at least 1/384 away from an integer, far above the rounding error.
+    if (nmax < 2147483648ull) {
+        // 2n <= 2^32, q <= U < 2^44.  q0 = nearest integer to the estimate
+        // (2^52 + qd leaves it in the low mantissa bits) is q or q + 1, so the
+        // true remainder y - q0*n lies in [-n, n) and its low 32 bits decide
+        // the correction exactly.  The line index off / 192 is taken in FP64:
+        // off is exact there (< 2^52) and (off + 1/2) / 192 is at least 1/384
+        // away from an integer, far above the rounding error.  Two positions
+        // per step share one reciprocal.
         const uint32_t ylo = (uint32_t)y;
-        const uint32_t nb = (uint32_t)nlo;
         const u64 base = (u64)tp.ud + 1;
         const double cbase = 4503599627370496.0 + (double)base;   // 2^52 + base, exact
-        for (uint32_t i = lane; i < (uint32_t)cnt; i += 64) {
-            const uint32_t n = nb + (uint32_t)ST * i;
-            const double qm = __builtin_fma(dy, rcp_fast((double)n), 4503599627370496.0);
+        const uint32_t iters = lane_iters(cnt, lane);
+        uint32_t n = (uint32_t)nlo + (uint32_t)(ST * lane);
+        constexpr uint32_t D = 64u * ST;
+        auto lookup = [&](uint32_t nn, double rn) -> int32_t {
+            const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
             const u64 q0 = (u64)__double_as_longlong(qm) & 0xfffffffffffffull;
-            const uint32_t r = ylo - (uint32_t)q0 * n;
-            const bool over = r >= 2u * n;                 // q0 = q + 1
+            const uint32_t r = ylo - (uint32_t)q0 * nn;
+            const bool over = r >= nn;                     // q0 = q + 1
             const u64 off = q0 - base - (over ? 1ull : 0ull);
             const double offd = (qm - cbase) - (over ? 1.0 : 0.0);
             const uint32_t line = (uint32_t)__builtin_floor(
                 __builtin_fma(offd, 1.0 / LINE_VALUES, 0.5 / LINE_VALUES));
             const uint32_t sub = (uint32_t)(off >> 6) - 3u * line;
-            s += line_M(tp.cl + line, sub, (uint32_t)(off & 63));
+            return line_M(tp.cl + line, sub, (uint32_t)(off & 63));
+        };
+        uint32_t k = 0;
+        for (; k + 2 <= iters; k += 2) {
+            double r1, r2;
+            rcp_pair((double)n, (double)(n + D), r1, r2);
+            s += lookup(n, r1);
+            s += lookup(n + D, r2);
+            n += 2 * D;
         }
+        if (k < iters) s += lookup(n, rcp_fast((double)n));
         return s;
     }
     for (u64 i = lane; i < cnt; i += 64) {
@@ -228,11 +261,11 @@ __device__ __forceinline__ uint32_t piece32(uint32_t q) {
 template <int P>
 __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane) {
     const double dy = (double)y;
-    // j >= (y >> 32) + 1 gives q < 2^32; the 32-bit remainder test also
-    // needs 3j <= 2^32.
-    const i64 j32 = (i64)(y >> 32) + 1;
+    // j >= (y >> 32) + 2 gives q <= 2^32 - 2; the 32-bit remainder test also
+    // needs 2j <= 2^32 (qdiv32n).
+    const i64 j32 = (i64)(y >> 32) + 2;
     u64 t_split = (u64)count_coprime6(j32 - 1);
-    if (j_of_t((i64)t_hi - 1) >= 1431655765ll) t_split = t_hi;
+    if (j_of_t((i64)t_hi - 1) >= 2147483648ll) t_split = t_hi;
     const u64 ts = t_split < t_lo ? t_lo : (t_split > t_hi ? t_hi : t_split);
     i64 s = 0;
     for (u64 t = t_lo + lane; t < ts; t += 64) {
@@ -244,24 +277,37 @@ __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane)
     }
     // q < 2^32: with code = mu + 1 in {0, 1, 2}, sum mu*v = sum code*v - sum v
     // (one multiply-add per position); piece 4 can be negative and takes the
-    // signed form.
+    // signed form.  A lane's t advances by 64, so j by 192, its mu word by 4
+    // and the code's shift stays; two positions per step share one reciprocal.
     const uint32_t ylo = (uint32_t)y;
-    const uint32_t cnt = (uint32_t)(t_hi - ts);
-    const uint32_t tb = (uint32_t)ts;
     u64 acc_c = 0, acc_v = 0;
     i64 acc_s = 0;
-    for (uint32_t i = lane; i < cnt; i += 64) {
-        const uint32_t t = tb + i;
-        const uint32_t j = 3u * t + 1u + (t & 1u);
-        const uint32_t code = (tp.mu6[t >> 4] >> (2 * (t & 15))) & 3u;
-        const uint32_t q = qdiv32(ylo, dy, j, rcp_fast((double)j));
-        const uint32_t v = piece32<P>(q);
-        if constexpr (P == 4) {
-            acc_s += (i64)((int)code - 1) * (i64)(int32_t)v;
-        } else {
-            acc_c += (u64)code * (u64)v;
-            acc_v += (u64)v;
+    const uint32_t iters = lane_iters(t_hi - ts, lane);
+    if (iters) {
+        const uint32_t t0 = (uint32_t)ts + (uint32_t)lane;
+        const uint32_t* wp = tp.mu6 + (t0 >> 4);
+        const uint32_t sh = 2u * (t0 & 15u);
+        uint32_t j = 3u * t0 + 1u + (t0 & 1u);
+        auto add = [&](uint32_t code, uint32_t v) {
+            if constexpr (P == 4) {
+                acc_s += (i64)((int)code - 1) * (i64)(int32_t)v;
+            } else {
+                acc_c += (u64)code * (u64)v;
+                acc_v += (u64)v;
+            }
+        };
+        uint32_t k = 0;
+        for (; k + 2 <= iters; k += 2) {
+            double r1, r2;
+            rcp_pair((double)j, (double)(j + 192u), r1, r2);
+            const uint32_t q1 = qdiv32n(ylo, dy, j, r1);
+            const uint32_t q2 = qdiv32n(ylo, dy, j + 192u, r2);
+            add((wp[0] >> sh) & 3u, piece32<P>(q1));
+            add((wp[4] >> sh) & 3u, piece32<P>(q2));
+            j += 384u;
+            wp += 8;
         }
+        if (k < iters) add((wp[0] >> sh) & 3u, piece32<P>(qdiv32n(ylo, dy, j, rcp_fast((double)j))));
     }
     return s + (i64)(acc_c - acc_v) + acc_s;
 }
