@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "tune_cache.txt"))
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--report", default=None, help="also write a JSON report here")
+    ap.add_argument("--merge", default=None, help="keep this cache's lines for decades not tuned now")
+    ap.add_argument("--no-batch", action="store_true", help="keep the batch factor of --merge")
     args = ap.parse_args()
     a, b = (int(v) for v in args.decades.split("-"))
     E = mdr_hip.Engine(0)
@@ -144,8 +146,20 @@ def main():
         cfg, t = tune_decade(T, d, waves, args.quick)
         rows.append((d, cfg))
         report["decades"][d] = dict(cfg, ms=round(t * 1000, 3))
-    fb, tb = tune_batch(E, 3, 4096, 10**9, 10**14, args.quick)
-    report["batch"] = {"fb": fb, "ms": round(tb * 1000, 2)}
+    kept, fb = {}, None
+    if args.merge and os.path.exists(args.merge):
+        for line in open(args.merge):
+            f = line.split()
+            if not f or f[0].startswith("#"):
+                continue
+            if f[0] == "batch":
+                fb = float(f[1])
+            elif int(f[0]) not in range(a, b + 1):
+                kept[int(f[0])] = {"fx": float(f[1]), "cx": float(f[2]), "chunks": int(f[3]), "ud": int(f[4])}
+    rows = sorted(list(kept.items()) + rows)
+    if fb is None or not args.no_batch:
+        fb, tb = tune_batch(E, 3, 4096, 10**9, 10**14, args.quick)
+        report["batch"] = {"fb": fb, "ms": round(tb * 1000, 2)}
     lines = [f"# written by autotune.py on {time.strftime('%Y-%m-%d %H:%M:%S')}, {waves} resident quotient waves",
              "# decade fx cx chunks u_dense"]
     lines += [f"{d} {c['fx']} {c['cx']} {c['chunks']} {c['ud']}" for d, c in rows]

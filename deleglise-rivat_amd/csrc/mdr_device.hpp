@@ -139,10 +139,16 @@ __device__ __forceinline__ int mu6_code(const uint32_t* mu6, i64 t) {
 }
 
 // M from one compressed line: word sub (0..2), bit b (0..63) of the word.
+// The prefix popcount over bits 0..b is the popcount of the word shifted left
+// by 63 - b (the bits above b leave the word), which needs no mask.
 __device__ __forceinline__ int32_t line_M(const CLine* L, uint32_t sub, uint32_t b) {
-    const u64 nz = L->pl[2 * sub], ng = L->pl[2 * sub + 1];
-    const u64 mask = ~0ull >> (63 - b);
-    return L->ck[sub] + (int)__popcll(nz & mask) - 2 * (int)__popcll(ng & mask);
+    const char* la = reinterpret_cast<const char*>(L);
+    const uint4 pw = *reinterpret_cast<const uint4*>(la + (sub << 4));
+    const int32_t ck = *reinterpret_cast<const int32_t*>(la + 48 + (sub << 2));
+    const uint32_t sh = 63u - (b & 63u);
+    const u64 nz = ((u64)pw.y << 32 | pw.x) << sh;
+    const u64 ng = ((u64)pw.w << 32 | pw.z) << sh;
+    return ck + (int)__popcll(nz) - 2 * (int)__popcll(ng);
 }
 
 // M(q) in the compressed tier (q > ud), off = q - ud - 1 < 192 * 2^32: the

@@ -217,15 +217,27 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
             const uint32_t sub = (uint32_t)(off >> 6) - 3u * line;
             return line_M(tp.cl + line, sub, (uint32_t)(off & 63));
         };
-        uint32_t k = 0;
-        for (; k + 2 <= iters; k += 2) {
-            double r1, r2;
-            rcp_pair((double)n, (double)(n + D), r1, r2);
-            s += lookup(n, r1);
-            s += lookup(n + D, r2);
-            n += 2 * D;
+        // int32 partial sums over at most 2^11 positions: the table holds
+        // q <= U < 192 * 2^32 < 2^40, where |M(q)| < 0.6 sqrt(q) < 2^19.3
+        // (|M(x)| / sqrt(x) stays below 0.6 through x = 10^16), so each
+        // partial sum stays below 2^30.3
+        for (uint32_t k0 = 0; k0 < iters; k0 += 2048u) {
+            const uint32_t kend = iters - k0 < 2048u ? iters : k0 + 2048u;
+            int32_t part = 0;
+            uint32_t k = k0;
+            for (; k + 2 <= kend; k += 2) {
+                double r1, r2;
+                rcp_pair((double)n, (double)(n + D), r1, r2);
+                part += lookup(n, r1);
+                part += lookup(n + D, r2);
+                n += 2 * D;
+            }
+            if (k < kend) {
+                part += lookup(n, rcp_fast((double)n));
+                n += D;
+            }
+            s += part;
         }
-        if (k < iters) s += lookup(n, rcp_fast((double)n));
         return s;
     }
     for (u64 i = lane; i < cnt; i += 64) {
