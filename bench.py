@@ -80,13 +80,19 @@ def roofline(stats):
     ms = sum(s.quotient_ms for s in stats) / len(stats)
     sec = ms / 1000.0
     ach = byts / sec / 1e9
-    traffic = None
+    traffic, issue = None, {}
     tf = os.path.join(ROOT, "profiles", "traffic_k_quot_1e17.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        prof = json.load(open(tf))
+        traffic = prof.get("hbm_bytes_per_launch")
+        issue = prof.get("issue", {}).get("k_quot", {})
     return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_quot",
-            "algorithmic_bytes": byts, "kernel_ms": round(ms, 3), "launches": len(stats)}
+            "algorithmic_bytes": byts, "kernel_ms": round(ms, 3), "launches": len(stats),
+            "valu_issue_frac": issue.get("valu_issue_frac"), "l2_hit_rate": issue.get("l2_hit_rate"),
+            "note": "per-unit bytes count every compressed lookup as a 20-byte HBM fetch; L2/MALL serve part of "
+                    "them (traffic, from FETCH/WRITE_SIZE, is below the algorithmic bytes), so frac can exceed 1; "
+                    "the binding resource is VALU issue (valu_issue_frac, 4 cycles per wave64 op, a lower bound)"}
 
 
 def main():
