@@ -38,14 +38,13 @@ __device__ __forceinline__ int8_t finalize(uint8_t a, u64 n) {
 }  // namespace
 
 __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
-    __shared__ uint32_t acc32[SEG / 4];
-    __shared__ uint32_t nib32[SEG / 16];   // per dword: nonzero nibble | negative nibble << 4
+    __shared__ __align__(16) uint32_t acc32[SEG / 4];
+    __shared__ __align__(16) uint32_t nib32[SEG / 16];   // per dword: nonzero nibble | negative nibble << 4
     __shared__ int32_t gsum[GROUPS];
     __shared__ int32_t gpre[GROUPS];
     __shared__ uint32_t small_off[64];
     __shared__ uint32_t sq_off[64];
     uint8_t* acc = reinterpret_cast<uint8_t*>(acc32);
-    uint8_t* nib = reinterpret_cast<uint8_t*>(nib32);
 
     const i64 s = (i64)blockIdx.x + a.seg_first;
     const int tid = threadIdx.x;
@@ -106,12 +105,14 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
     // 4. primes 256 <= p <= sqrt(last n), one per thread, four in flight.
     //    r2 = first mod p^2 gives the (at most one) square hit, r2 mod p the
-    //    first weight hit; both in FP64, exact: first < 2^53, the quotient
-    //    estimates are within 1, and fma(-q, m, x) is the exact remainder
-    //    (|x - q m| < 2m is representable).  Primes ascend, so a thread stops
-    //    at the first p with p^2 beyond the segment.
+    //    first weight hit; both in FP64, exact: first < 2^53, the rounded
+    //    quotient estimates are within 1/2 + 2^-20 of the true quotients, and
+    //    fma(-q, m, x) is the exact remainder (|x - q m| < m is representable).
+    //    Primes ascend, so a thread stops at the first p with p^2 beyond the
+    //    segment.
     constexpr int UNR = 4;
     const double dfirst = (double)first;
+    const double dlen = (double)len;
     for (int i0 = a.n_small + tid; i0 < a.n_w; i0 += UNR * SIEVE_THREADS) {
         uint32_t pp[UNR];
 #pragma unroll
@@ -126,16 +127,17 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
             if ((u64)p * p > lastn) continue;
             const double dp = (double)p, dp2 = dp * dp;
             const double rp = rcp_fast(dp);
-            const double q2 = __builtin_floor(dfirst * (rp * rp));
-            double r2 = __builtin_fma(-q2, dp2, dfirst);
-            if (r2 < 0) r2 += dp2;
-            else if (r2 >= dp2) r2 -= dp2;
-            const u64 xs = r2 != 0.0 ? (u64)(dp2 - r2) : 0;
-            if (xs < (u64)len) atomicOr(&acc32[xs >> 2], 0x80u << (8 * (xs & 3)));
-            const double q1 = __builtin_floor(r2 * rp);
-            double r = __builtin_fma(-q1, dp, r2);
-            if (r < 0) r += dp;
-            else if (r >= dp) r -= dp;
+            // nearest-integer quotient estimates leave remainders in [-m, m):
+            // one correction each
+            double r2 = __builtin_fma(-__builtin_rint(dfirst * (rp * rp)), dp2, dfirst);
+            r2 += r2 < 0.0 ? dp2 : 0.0;
+            const double xs = r2 != 0.0 ? dp2 - r2 : 0.0;   // square hit offset
+            if (xs < dlen) {
+                const uint32_t x = (uint32_t)xs;
+                atomicOr(&acc32[x >> 2], 0x80u << (8 * (x & 3)));
+            }
+            double r = __builtin_fma(-__builtin_rint(r2 * rp), dp, r2);
+            r += r < 0.0 ? dp : 0.0;
             const uint32_t ri = (uint32_t)r;
             const uint32_t w = (32u - (uint32_t)__clz(p - 1u)) | 1u;
             for (uint32_t x = ri ? p - ri : 0; x < (uint32_t)len; x += p)
@@ -144,40 +146,65 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     }
     __syncthreads();
 
-    // 5. finalise to mu in {-1, 0, 1}, four values per dword (SWAR): a flag
-    //    gives 0; otherwise mu = +1 exactly when the parity bit differs from
-    //    [acc > floor(log2 n)].  Dwords at n = 1, 2, past U or across a power
-    //    of two take the per-byte path.  Each dword also leaves a byte of its
-    //    nonzero and negative bits (low/high nibble) for the plane builder.
-    for (int i = tid; i < SEG / 4; i += SIEVE_THREADS) {
-        const uint32_t x = acc32[i];
-        const u64 n0 = first + 4 * (u64)i;
-        const unsigned t0 = 63u - (unsigned)__clzll(n0);
-        uint32_t mu;
-        if (n0 > 2 && n0 + 3 <= (u64)a.U && t0 == 63u - (unsigned)__clzll(n0 + 3)) {
-            const uint32_t gt = ((x & 0x7f7f7f7fu) + (127u - t0) * 0x01010101u) & 0x80808080u;
+    // 5. finalise to mu in {-1, 0, 1}, 16 values per lane (four dwords, SWAR):
+    //    a flag gives 0; otherwise mu = +1 exactly when the parity bit differs
+    //    from [acc > floor(log2 n)].  floor(log2 n) is tseg below the next power
+    //    of two b and tseg + 1 from b on (segments from 2^17 on are shorter than
+    //    b); the group holding b - 1 and b, groups reaching past U, n = 1, 2 and
+    //    the first segments take the per-byte path.  Each dword also leaves a
+    //    byte of its nonzero and negative bits (low/high nibble) for the plane
+    //    builder.
+    {
+        const unsigned tseg = 63u - (unsigned)__clzll(first);
+        const u64 db = (2ull << tseg) - first;                  // b - first >= 1
+        const uint32_t gb = db >= (u64)SEG ? 0xffffffffu : (uint32_t)(db >> 4);
+        const bool straddle = (db & 15) != 0;
+        const uint32_t gend = (uint32_t)len >> 4;               // groups wholly <= U
+        const bool general = first < (1ull << 17);
+        const uint32_t add_lo = (127u - tseg) * 0x01010101u, add_hi = (126u - tseg) * 0x01010101u;
+        uint4* acc4 = reinterpret_cast<uint4*>(acc32);
+        auto swar = [](uint32_t x, uint32_t add) {
+            const uint32_t gt = ((x & 0x7f7f7f7fu) + add) & 0x80808080u;
             const uint32_t t = (x & 0x01010101u) ^ (gt >> 7);
-            mu = (0xffffffffu ^ (t * 0xfeu)) & ~(((x >> 7) & 0x01010101u) * 0xffu);
-        } else {
-            mu = 0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const u64 n = n0 + b;
-                int8_t m = 0;
-                if (n <= (u64)a.U) {
-                    m = finalize((uint8_t)(x >> (8 * b)), n);
-                    if (n == 1) m = 1;
-                    if (n == 2) m = -1;
-                }
-                mu |= (uint32_t)(uint8_t)m << (8 * b);
-            }
-        }
-        acc32[i] = mu;
+            return (0xffffffffu ^ (t * 0xfeu)) & ~(((x >> 7) & 0x01010101u) * 0xffu);
+        };
         // bits at 0, 8, 16, 24 gathered to bits 0..3 (the four partial
         // products of the multiply land on distinct bits, so nothing carries)
-        const uint32_t nzn = (((mu & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
-        const uint32_t ngn = ((((mu >> 7) & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
-        nib[i] = (uint8_t)(nzn | (ngn << 4));
+        auto nibs = [](uint32_t mu) {
+            const uint32_t nzn = (((mu & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
+            const uint32_t ngn = ((((mu >> 7) & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
+            return nzn | (ngn << 4);
+        };
+        for (uint32_t g = tid; g < SEG / 16; g += SIEVE_THREADS) {
+            uint4 v = acc4[g];
+            if (!general && g < gend && !(straddle && g == gb)) {
+                const uint32_t add = g < gb ? add_lo : add_hi;
+                v.x = swar(v.x, add);
+                v.y = swar(v.y, add);
+                v.z = swar(v.z, add);
+                v.w = swar(v.w, add);
+            } else {
+                uint32_t* w = &v.x;
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    uint32_t mu = 0;
+#pragma unroll
+                    for (int bb = 0; bb < 4; ++bb) {
+                        const u64 n = first + 16 * (u64)g + 4 * d + bb;
+                        int8_t m = 0;
+                        if (n <= (u64)a.U) {
+                            m = finalize((uint8_t)(w[d] >> (8 * bb)), n);
+                            if (n == 1) m = 1;
+                            if (n == 2) m = -1;
+                        }
+                        mu |= (uint32_t)(uint8_t)m << (8 * bb);
+                    }
+                    w[d] = mu;
+                }
+            }
+            acc4[g] = v;
+            nib32[g] = nibs(v.x) | (nibs(v.y) << 8) | (nibs(v.z) << 16) | (nibs(v.w) << 24);
+        }
     }
     __syncthreads();
 
