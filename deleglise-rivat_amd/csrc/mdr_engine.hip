@@ -394,7 +394,8 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     a.cl = c->cl.as<CLine>();
     a.mu6 = c->mu6.as<uint32_t>();
     a.segsum = c->segsum.as<i64>();
-    const i64 maxgrid = 1 << 30;
+    // a dispatch holds fewer than 2^32 work-items: at most 2^22 blocks of 512
+    const i64 maxgrid = 1 << 22;
     for (i64 s0 = 0; s0 < j.nseg; s0 += maxgrid) {
         a.seg_first = s0;
         const i64 nb = std::min(maxgrid, j.nseg - s0);
@@ -411,7 +412,8 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     f.cl = c->cl.as<CLine>();
     f.segoff = c->segsum.as<i64>();
     const i64 nfix = std::max<i64>(1, std::max(j.ud, j.NL));
-    hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)((nfix + 255) / 256)), dim3(256), 0, c->st, f);
+    const i64 fix_blocks = std::min<i64>((nfix + 255) / 256, 1 << 20);   // grid-stride kernel
+    hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)fix_blocks), dim3(256), 0, c->st, f);
     HIPT(hipGetLastError(), "launch fixup");
     return MDR_OK;
 }
@@ -1006,8 +1008,12 @@ mdr_status run_component(mdr_ctx* c, const Seg& sg, i64 y, int geval, int comb2,
 
 extern "C" {
 
+// One dispatch holds fewer than 2^32 work-items: reads and model batches are
+// limited to 2^31 entries per call.
+constexpr long long MAX_CALL_ITEMS = 1LL << 31;
+
 mdr_status mdr_table_read_M(mdr_ctx* c, long long first, long long count, int32_t* out) {
-    if (!c || count < 0 || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    if (!c || count < 0 || count > MAX_CALL_ITEMS || (count && !out) || first < 0) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
     if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
     if (first + count - 1 > c->tU) return fail(c, MDR_ERR_INVALID, "read beyond the table");
@@ -1025,7 +1031,7 @@ mdr_status mdr_table_read_M(mdr_ctx* c, long long first, long long count, int32_
 }
 
 mdr_status mdr_table_read_mu(mdr_ctx* c, long long first, long long count, int8_t* out) {
-    if (!c || count < 0 || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    if (!c || count < 0 || count > MAX_CALL_ITEMS || (count && !out) || first < 0) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
     if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
     if (count == 0) return MDR_OK;
@@ -1199,7 +1205,7 @@ mdr_status mdr_mlp_features(const mdr_stamp* s, size_t n, float* feats, float* t
 }
 
 mdr_status mdr_mlp_predict(mdr_ctx* c, const float* w, const float* feats, size_t n, float* out) {
-    if (!c || !w || (n && (!feats || !out))) return MDR_ERR_INVALID;
+    if (!c || !w || (n && (!feats || !out)) || n > (size_t)MAX_CALL_ITEMS) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
     if (n == 0) return MDR_OK;
     DevTmp dw, dx, dy;
@@ -1216,7 +1222,7 @@ mdr_status mdr_mlp_predict(mdr_ctx* c, const float* w, const float* feats, size_
 
 mdr_status mdr_mlp_grad(mdr_ctx* c, const float* w, const float* feats, const float* targets, size_t n, float* grad,
                         float* loss) {
-    if (!c || !w || !feats || !targets || !grad || n == 0) return MDR_ERR_INVALID;
+    if (!c || !w || !feats || !targets || !grad || n == 0 || n > (size_t)MAX_CALL_ITEMS) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
     DevTmp dw, dx, dt, buf, err, dg, dl;
     HIPT(dw.alloc(sizeof(float) * MLP_NPARAM), "alloc");
@@ -1242,7 +1248,8 @@ mdr_status mdr_mlp_grad(mdr_ctx* c, const float* w, const float* feats, const fl
 
 mdr_status mdr_mlp_train(mdr_ctx* c, const float* feats, const float* targets, size_t n, int epochs, int batch,
                          float lr, unsigned int seed, float* final_loss) {
-    if (!c || !feats || !targets || n == 0 || epochs < 1 || batch < 1 || !(lr > 0)) return MDR_ERR_INVALID;
+    if (!c || !feats || !targets || n == 0 || n > (size_t)MAX_CALL_ITEMS || epochs < 1 || batch < 1 || !(lr > 0))
+        return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
     std::vector<float> w0 = c->mlp_host;
     std::mt19937 rng(seed);

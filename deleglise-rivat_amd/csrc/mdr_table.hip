@@ -251,14 +251,13 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     __syncthreads();
 
     // 7. checkpoints of compressed words (segment-local; the fix-up adds the
-    //    segment offset) and dense prefix values, one lane per dword
+    //    segment offset) and dense prefix values
     for (int k = tid; k < GROUPS; k += SIEVE_THREADS) {
         const u64 n0 = first + (u64)k * 64;
         if (n0 > (u64)a.ud && n0 <= (u64)a.U) {
             const u64 w = (n0 - (u64)a.ud - 1) >> 6;
             const u64 line = w / 3u;
-            const uint32_t sub = (uint32_t)(w - 3u * line);
-            a.cl[line].ck[sub] = gpre[k];
+            a.cl[line].ck[(uint32_t)(w - 3u * line)] = gpre[k];
         }
     }
     if (first <= (u64)a.ud) {
@@ -303,15 +302,18 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
 
 // Adds the exclusive segment offsets to the segment-local values and sets
 // M(0) = 0.
+// Grid-stride: a dispatch holds fewer than 2^32 work-items, the words can
+// be more.
 __global__ void k_table_fixup(FixupArgs a) {
-    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) a.Md[0] = 0;
-    if (i < a.ud) {
+    const i64 i0 = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    if (i0 == 0) a.Md[0] = 0;
+    for (i64 i = i0; i < a.ud; i += stride) {
         const i64 n = i + 1;
         const i64 s = (n - 1) / SEG;
         a.Md[n] = (int16_t)(a.Md[n] + (int32_t)a.segoff[s]);
     }
-    if (i < a.L) {
+    for (i64 i = i0; i < a.L; i += stride) {   // line i: its three checkpoints
         const int32_t add = (int32_t)a.segoff[(a.ud + (i64)LINE_VALUES * i) / SEG];
         int32_t* ck = a.cl[i].ck;
         ck[0] += add;
