@@ -134,8 +134,13 @@ __host__ __device__ __forceinline__ i64 count_coprime6(i64 x) {
 }
 __host__ __device__ __forceinline__ i64 j_of_t(i64 t) { return 3 * t + 1 + (t & 1); }
 
-__device__ __forceinline__ int mu6_code(const uint32_t* mu6, i64 t) {
-    return (int)((mu6[t >> 4] >> (2 * (t & 15))) & 3u);
+// mu(j(t)) from its 2-bit two's-complement code (1 = 01, 0 = 00, -1 = 11):
+// a signed bit-field extract.
+__device__ __forceinline__ int mu6_field(uint32_t word, uint32_t shift) {
+    return (int)(word << (30u - shift)) >> 30;
+}
+__device__ __forceinline__ int mu6_mu(const uint32_t* mu6, i64 t) {
+    return mu6_field(mu6[t >> 4], (uint32_t)(2 * (t & 15)));
 }
 
 // M from one compressed line: word sub (0..2), bit b (0..63) of the word.
@@ -204,7 +209,7 @@ __device__ inline Frame make_frame(const XParam& xp, const uint32_t* mu6, i64 g)
     const i64 t = g >> 1;
     const int half = (int)(g & 1);
     const i64 k = j_of_t(t);
-    const int muk = mu6_code(mu6, t) - 1;
+    const int muk = mu6_mu(mu6, t);
     if (muk == 0 || k > xp.N) return f;
     const int range = k <= xp.n6 ? 1 : k <= xp.n3 ? 2 : k <= xp.n2 ? 3 : 4;
     i64 y = div_exact(xp.X, k);

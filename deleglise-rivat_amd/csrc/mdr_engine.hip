@@ -933,7 +933,7 @@ __global__ void k_read_mu(TableParam tp, i64 first, i64 count, int8_t* out) {
         else if (v % 2 == 0) { v /= 2; sgn = -sgn; }
         if (v % 9 == 0) zero = true;
         else if (v % 3 == 0) { v /= 3; sgn = -sgn; }
-        if (!zero) m = (int8_t)(sgn * (mu6_code(tp.mu6, v / 3) - 1));
+        if (!zero) m = (int8_t)(sgn * mu6_mu(tp.mu6, v / 3));
     }
     out[i] = m;
 }

@@ -273,55 +273,43 @@ __device__ __forceinline__ uint32_t piece32(uint32_t q) {
 template <int P>
 __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane) {
     const double dy = (double)y;
-    // j >= (y >> 32) + 2 gives q <= 2^32 - 2; the 32-bit remainder test also
-    // needs 2j <= 2^32 (qdiv32n).
-    const i64 j32 = (i64)(y >> 32) + 2;
-    u64 t_split = (u64)count_coprime6(j32 - 1);
-    if (j_of_t((i64)t_hi - 1) >= 2147483648ll) t_split = t_hi;
-    const u64 ts = t_split < t_lo ? t_lo : (t_split > t_hi ? t_hi : t_split);
+    // j >= (y >> 31) + 2 gives q <= 2^31 - 2, so every summand fits int32 and
+    // mu * v is one signed multiply-add into 64 bits; the 32-bit remainder
+    // test also needs 2j <= 2^32 (qdiv32n).  Smaller j (few positions) take
+    // the 64-bit path.
+    u64 ts = (u64)count_coprime6((i64)(y >> 31) + 1);
+    if (j_of_t((i64)t_hi - 1) >= 2147483648ll) ts = t_hi;
+    ts = ts < t_lo ? t_lo : (ts > t_hi ? t_hi : ts);
     i64 s = 0;
     for (u64 t = t_lo + lane; t < ts; t += 64) {
         const u64 j = (u64)j_of_t((i64)t);
-        const int code = mu6_code(tp.mu6, (i64)t);
         const u64 q = qdiv_big(y, dy, j, rcp_fast((double)j));
-        const i64 v = piece_eval<P, u64>(q);
-        s += code == 2 ? v : (code == 0 ? -v : 0);
+        s += (i64)mu6_mu(tp.mu6, (i64)t) * piece_eval<P, u64>(q);
     }
-    // q < 2^32: with code = mu + 1 in {0, 1, 2}, sum mu*v = sum code*v - sum v
-    // (one multiply-add per position); piece 4 can be negative and takes the
-    // signed form.  A lane's t advances by 64, so j by 192, its mu word by 4
-    // and the code's shift stays; two positions per step share one reciprocal.
-    const uint32_t ylo = (uint32_t)y;
-    u64 acc_c = 0, acc_v = 0;
-    i64 acc_s = 0;
+    // A lane's t advances by 64, so j by 192, its mu word by 4 and the
+    // field's shift stays; two positions per step share one reciprocal.
     const uint32_t iters = lane_iters(t_hi - ts, lane);
     if (iters) {
+        const uint32_t ylo = (uint32_t)y;
         const uint32_t t0 = (uint32_t)ts + (uint32_t)lane;
         const uint32_t* wp = tp.mu6 + (t0 >> 4);
         const uint32_t sh = 2u * (t0 & 15u);
         uint32_t j = 3u * t0 + 1u + (t0 & 1u);
-        auto add = [&](uint32_t code, uint32_t v) {
-            if constexpr (P == 4) {
-                acc_s += (i64)((int)code - 1) * (i64)(int32_t)v;
-            } else {
-                acc_c += (u64)code * (u64)v;
-                acc_v += (u64)v;
-            }
-        };
         uint32_t k = 0;
         for (; k + 2 <= iters; k += 2) {
             double r1, r2;
             rcp_pair((double)j, (double)(j + 192u), r1, r2);
             const uint32_t q1 = qdiv32n(ylo, dy, j, r1);
             const uint32_t q2 = qdiv32n(ylo, dy, j + 192u, r2);
-            add((wp[0] >> sh) & 3u, piece32<P>(q1));
-            add((wp[4] >> sh) & 3u, piece32<P>(q2));
+            s += (i64)mu6_field(wp[0], sh) * (i64)(int32_t)piece32<P>(q1);
+            s += (i64)mu6_field(wp[4], sh) * (i64)(int32_t)piece32<P>(q2);
             j += 384u;
             wp += 8;
         }
-        if (k < iters) add((wp[0] >> sh) & 3u, piece32<P>(qdiv32n(ylo, dy, j, rcp_fast((double)j))));
+        if (k < iters)
+            s += (i64)mu6_field(wp[0], sh) * (i64)(int32_t)piece32<P>(qdiv32n(ylo, dy, j, rcp_fast((double)j)));
     }
-    return s + (i64)(acc_c - acc_v) + acc_s;
+    return s;
 }
 
 __device__ i64 run_seg(const TableParam& tp, const Seg& sg, u64 y, u64 p_lo, u64 p_hi, int lane) {

@@ -283,7 +283,8 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
         }
     }
 
-    // 8. packed mu for j coprime to 6 (t = j/3; 16 codes per word)
+    // 8. packed mu for j coprime to 6 (t = j/3; 16 two's-complement 2-bit
+    //    codes per word)
     if (s < a.mu_segs) {
         const i64 w0 = s * (SEG / 48);
         for (int wi = tid; wi < SEG / 48; wi += SIEVE_THREADS) {
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
                 const int tl = wi * 16 + e;                // local t
                 const int jl = 3 * tl + (tl & 1);          // j - base - 1
                 const int m = (int8_t)acc[jl];
-                word |= (uint32_t)(m + 1) << (2 * e);
+                word |= ((uint32_t)m & 3u) << (2 * e);
             }
             a.mu6[w0 + wi] = word;
         }
