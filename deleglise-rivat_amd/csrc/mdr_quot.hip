@@ -346,6 +346,41 @@ __device__ __forceinline__ i64 uniform64(i64 v) {
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((u64)v >> 32));
     return (i64)(((u64)hi << 32) | lo);
 }
+__device__ __forceinline__ int uniform32(int v) { return (int)__builtin_amdgcn_readfirstlane((uint32_t)v); }
+// Frame and segment values are the same in every lane of a wave: reading
+// them back from lane 0 lets the compiler keep them in scalar registers
+// (fewer vector registers, more resident waves).
+__device__ __forceinline__ Frame uniform_frame(Frame f) {
+    f.y = uniform64(f.y);
+    f.A = uniform64(f.A);
+    f.B = uniform64(f.B);
+    f.ky = uniform64(f.ky);
+    f.ky2 = uniform64(f.ky2);
+    f.comb2 = uniform32(f.comb2);
+    f.coef = uniform32(f.coef);
+    return f;
+}
+__device__ __forceinline__ XParam uniform_xp(XParam p) {
+    p.X = uniform64(p.X);
+    p.u = uniform64(p.u);
+    p.N = uniform64(p.N);
+    p.T = uniform64(p.T);
+    p.slot_base = uniform64(p.slot_base);
+    p.cx = __longlong_as_double(uniform64(__double_as_longlong(p.cx)));
+    p.n6 = uniform64(p.n6);
+    p.n3 = uniform64(p.n3);
+    p.n2 = uniform64(p.n2);
+    return p;
+}
+__device__ __forceinline__ Seg uniform_seg(Seg s) {
+    s.type = uniform32(s.type);
+    s.piece = uniform32(s.piece);
+    s.sign = uniform32(s.sign);
+    s.first = uniform64(s.first);
+    s.stride = uniform64(s.stride);
+    s.count = uniform64(s.count);
+    return s;
+}
 __device__ __forceinline__ i64 fetch_next(unsigned long long* ctr, int lane) {
     unsigned long long v = 0;
     if (lane == 0) v = atomicAdd(ctr, 1ull);
@@ -370,21 +405,21 @@ __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
     XParam xp{};
     for (i64 c = a.dynamic ? fetch_next(a.next, lane) : uniform64(wave); c < a.P;
          c = a.dynamic ? fetch_next(a.next, lane) : c + nwaves) {
-        const i64 b0 = a.bounds[c], b1 = a.bounds[c + 1];
+        const i64 b0 = uniform64(a.bounds[c]), b1 = uniform64(a.bounds[c + 1]);
         if (b0 >= T) break;   // bounds ascend: every later chunk is empty too
         if (b0 >= b1) continue;
-        for (i64 G = last_le(a.off, a.nslots, b0); G < a.nslots; ++G) {
-            const i64 og = a.off[G];
+        for (i64 G = uniform64(last_le(a.off, a.nslots, b0)); G < a.nslots; ++G) {
+            const i64 og = uniform64(a.off[G]);
             if (og >= b1) break;
-            const i64 og1 = G + 1 < a.nslots ? a.off[G + 1] : T;
+            const i64 og1 = uniform64(G + 1 < a.nslots ? a.off[G + 1] : T);
             if (og1 <= og) continue;
             if (G >= x_end || cur_x < 0) {
                 if (cur_x >= 0) flush(a.out128, cur_x, acc_lo, acc_hi, lane);
-                cur_x = find_x(a.xp, a.nx, G);
-                xp = a.xp[cur_x];
-                x_end = cur_x + 1 < a.nx ? a.xp[cur_x + 1].slot_base : a.nslots;
+                cur_x = uniform64(find_x(a.xp, a.nx, G));
+                xp = uniform_xp(a.xp[cur_x]);
+                x_end = uniform64(cur_x + 1 < a.nx ? a.xp[cur_x + 1].slot_base : a.nslots);
             }
-            const Frame f = make_frame(xp, a.tp.mu6, G - xp.slot_base);
+            const Frame f = uniform_frame(make_frame(xp, a.tp.mu6, G - xp.slot_base));
             const i64 lo = (b0 > og ? b0 : og) - og, hi = (b1 < og1 ? b1 : og1) - og;
             if (lo == 0 && lane == 0) {
                 const i128 cst = (i128)f.coef * frame_const(f, a.tp);
@@ -394,7 +429,7 @@ __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
             const int ns = frame_nseg(f);
             i64 s0 = 1;
             for (int i = 0; i < ns && s0 < hi; ++i) {
-                const Seg sg = frame_seg(f, xp, a.tp, i);
+                const Seg sg = uniform_seg(frame_seg(f, xp, a.tp, i));
                 const i64 s1 = s0 + sg.count;
                 const i64 l = lo > s0 ? lo : s0, h = hi < s1 ? hi : s1;
                 if (l < h) {
