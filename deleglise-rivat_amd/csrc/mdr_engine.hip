@@ -125,6 +125,21 @@ mdr_status hip_fail(mdr_ctx* c, hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hip_fail(c, e_, what); \
     } while (0)
 
+// Copies and clears go through the context's own (non-blocking) stream and
+// wait for it: nothing touches the legacy null stream, which must not depend
+// on a stream that another thread is capturing into a graph.
+hipError_t copy_sync(mdr_ctx* c, void* dst, const void* src, size_t n, hipMemcpyKind kind) {
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, n, kind, c->st);
+    return e == hipSuccess ? hipStreamSynchronize(c->st) : e;
+}
+hipError_t h2d(mdr_ctx* c, void* dst, const void* src, size_t n) { return copy_sync(c, dst, src, n, hipMemcpyHostToDevice); }
+hipError_t d2h(mdr_ctx* c, void* dst, const void* src, size_t n) { return copy_sync(c, dst, src, n, hipMemcpyDeviceToHost); }
+hipError_t clear_sync(mdr_ctx* c, void* dst, size_t n) {
+    hipError_t e = hipMemsetAsync(dst, 0, n, c->st);
+    return e == hipSuccess ? hipStreamSynchronize(c->st) : e;
+}
+
 // ---- parameters -------------------------------------------------------------
 // Reference rules: mertens_dr.hpp:914-923 (cx), :931-962 (dense cutoff),
 // :1401-1419 (streaming cutoff), :1451-1461 (dense tier and mu limit),
@@ -304,11 +319,11 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
     HIPT(c->wweights.ensure(ww.size() + 4), "alloc primes");
     HIPT(c->sqprimes.ensure(sq.size() * 4 + 4), "alloc primes");
     if (!wp.empty()) {
-        HIPT(hipMemcpy(c->wprimes.p, wp.data(), wp.size() * 4, hipMemcpyHostToDevice), "upload primes");
-        HIPT(hipMemcpy(c->wrcp.p, wr.data(), wr.size() * 8, hipMemcpyHostToDevice), "upload primes");
-        HIPT(hipMemcpy(c->wweights.p, ww.data(), ww.size(), hipMemcpyHostToDevice), "upload primes");
+        HIPT(h2d(c, c->wprimes.p, wp.data(), wp.size() * 4), "upload primes");
+        HIPT(h2d(c, c->wrcp.p, wr.data(), wr.size() * 8), "upload primes");
+        HIPT(h2d(c, c->wweights.p, ww.data(), ww.size()), "upload primes");
     }
-    if (!sq.empty()) HIPT(hipMemcpy(c->sqprimes.p, sq.data(), sq.size() * 4, hipMemcpyHostToDevice), "upload primes");
+    if (!sq.empty()) HIPT(h2d(c, c->sqprimes.p, sq.data(), sq.size() * 4), "upload primes");
     if (!c->stencil.p) {
         // value for n with n = x+1 (mod period): weights of 2,3,5,7,11 and the
         // square flags of 4, 9 and 25 (mertens_dr.hpp:784-793 uses 2..11, 4, 9).
@@ -322,7 +337,7 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
             st[x] = (uint8_t)a;
         }
         HIPT(c->stencil.ensure(st.size()), "alloc stencil");
-        HIPT(hipMemcpy(c->stencil.p, st.data(), st.size(), hipMemcpyHostToDevice), "upload stencil");
+        HIPT(h2d(c, c->stencil.p, st.data(), st.size()), "upload stencil");
     }
     c->primes_root = root;
     return MDR_OK;
@@ -714,16 +729,16 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     // ---- results ----
     const auto td0 = std::chrono::steady_clock::now();
     std::vector<u64> res(nx * 2);
-    if (nx) HIPT(hipMemcpy(res.data(), c->out128.p, nx * 16, hipMemcpyDeviceToHost), "download");
+    if (nx) HIPT(d2h(c, res.data(), c->out128.p, nx * 16), "download");
     unsigned long long ch[7] = {};
     if (b.nxs) {
         std::vector<u64> acc_h(b.nxs * 2);
-        HIPT(hipMemcpy(acc_h.data(), c->comp_out.p, b.nxs * 16, hipMemcpyDeviceToHost), "download");
+        HIPT(d2h(c, acc_h.data(), c->comp_out.p, b.nxs * 16), "download");
         for (size_t i = 0; i < b.nxs; ++i) {
             res[2 * xidx[i]] = acc_h[2 * i];
             res[2 * xidx[i] + 1] = acc_h[2 * i + 1];
         }
-        HIPT(hipMemcpy(ch, c->ctr.p, sizeof ch, hipMemcpyDeviceToHost), "download");
+        HIPT(d2h(c, ch, c->ctr.p, sizeof ch), "download");
     }
     for (size_t i = 0; i < nx; ++i) {
         if (plans[i].kind == 0) { out[i] = plans[i].value; continue; }
@@ -734,7 +749,7 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     if (b.cap) {
         const i64 n = std::min<i64>((i64)ch[6], b.cap);
         c->last_stamps.resize((size_t)n);
-        if (n) HIPT(hipMemcpy(c->last_stamps.data(), c->stamps.p, (size_t)n * sizeof(StampRec), hipMemcpyDeviceToHost),
+        if (n) HIPT(d2h(c, c->last_stamps.data(), c->stamps.p, (size_t)n * sizeof(StampRec)),
                     "download cycle log");
     }
     if (stats) {
@@ -1126,7 +1141,7 @@ mdr_status set_model(mdr_ctx* c, const float* w, int n) {
     for (int i = 0; i < n; ++i)
         if (!std::isfinite(w[i])) return fail(c, MDR_ERR_INVALID, "cost model weights must be finite");
     HIPT(c->mlp.ensure(sizeof(float) * n), "alloc cost model");
-    HIPT(hipMemcpy(c->mlp.p, w, sizeof(float) * n, hipMemcpyHostToDevice), "upload cost model");
+    HIPT(h2d(c, c->mlp.p, w, sizeof(float) * n), "upload cost model");
     c->mlp_host.assign(w, w + n);
     c->mlp_ok = true;
     return MDR_OK;
@@ -1212,8 +1227,8 @@ mdr_status mdr_mlp_predict(mdr_ctx* c, const float* w, const float* feats, size_
     HIPT(dw.alloc(sizeof(float) * MLP_NPARAM), "alloc");
     HIPT(dx.alloc(sizeof(float) * MLP_IN * n), "alloc");
     HIPT(dy.alloc(sizeof(float) * n), "alloc");
-    HIPT(hipMemcpy(dw.p, w, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice), "upload");
-    HIPT(hipMemcpy(dx.p, feats, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice), "upload");
+    HIPT(h2d(c, dw.p, w, sizeof(float) * MLP_NPARAM), "upload");
+    HIPT(h2d(c, dx.p, feats, sizeof(float) * MLP_IN * n), "upload");
     HIPT(mlp_predict(dw.as<float>(), dx.as<float>(), (i64)n, dy.as<float>(), c->st), "predict");
     HIPT(hipMemcpyAsync(out, dy.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->st), "download");
     HIPT(hipStreamSynchronize(c->st), "predict");
@@ -1232,9 +1247,9 @@ mdr_status mdr_mlp_grad(mdr_ctx* c, const float* w, const float* feats, const fl
     HIPT(err.alloc(sizeof(float) * n), "alloc");
     HIPT(dg.alloc(sizeof(float) * MLP_NPARAM), "alloc");
     HIPT(dl.alloc(sizeof(float)), "alloc");
-    HIPT(hipMemcpy(dw.p, w, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice), "upload");
-    HIPT(hipMemcpy(dx.p, feats, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice), "upload");
-    HIPT(hipMemcpy(dt.p, targets, sizeof(float) * n, hipMemcpyHostToDevice), "upload");
+    HIPT(h2d(c, dw.p, w, sizeof(float) * MLP_NPARAM), "upload");
+    HIPT(h2d(c, dx.p, feats, sizeof(float) * MLP_IN * n), "upload");
+    HIPT(h2d(c, dt.p, targets, sizeof(float) * n), "upload");
     HIPT(mlp_grad(dw.as<float>(), dx.as<float>(), dt.as<float>(), nullptr, (i64)n, buf.as<float>(), err.as<float>(),
                   dg.as<float>(), dl.as<float>(), c->st),
          "gradient");
@@ -1278,11 +1293,11 @@ mdr_status mdr_mlp_train(mdr_ctx* c, const float* feats, const float* targets, s
     HIPT(dv.alloc(sizeof(float) * MLP_NPARAM), "alloc");
     HIPT(didx.alloc(sizeof(int) * n), "alloc");
     HIPT(dl.alloc(sizeof(float)), "alloc");
-    HIPT(hipMemcpy(dw.p, w0.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice), "upload");
-    HIPT(hipMemcpy(dx.p, feats, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice), "upload");
-    HIPT(hipMemcpy(dt.p, targets, sizeof(float) * n, hipMemcpyHostToDevice), "upload");
-    HIPT(hipMemset(dm.p, 0, sizeof(float) * MLP_NPARAM), "clear");
-    HIPT(hipMemset(dv.p, 0, sizeof(float) * MLP_NPARAM), "clear");
+    HIPT(h2d(c, dw.p, w0.data(), sizeof(float) * MLP_NPARAM), "upload");
+    HIPT(h2d(c, dx.p, feats, sizeof(float) * MLP_IN * n), "upload");
+    HIPT(h2d(c, dt.p, targets, sizeof(float) * n), "upload");
+    HIPT(clear_sync(c, dm.p, sizeof(float) * MLP_NPARAM), "clear");
+    HIPT(clear_sync(c, dv.p, sizeof(float) * MLP_NPARAM), "clear");
     std::vector<int> idx(n);
     for (size_t i = 0; i < n; ++i) idx[i] = (int)i;
     int step = 0;
