@@ -140,3 +140,17 @@ def test_dropin_header_parameter_helpers(plan_probe, env):
         chmt = env.get("MERTENS_USE_CHMT")
         assert int(f[5]) == (1 if chmt and chmt[:1] in "1yYtT" else 0)
         assert int(f[4]) == (1 if (X >= 10**11 and "MERTENS_FX" not in env and "MERTENS_U_CAP" not in env) else 0)
+
+
+def test_cli_usage_and_no_device():
+    # the command-line front end (bench_dr / bench_cuda modes): usage text
+    # without arguments, a clear error (exit 3) when no GPU is present
+    exe = os.path.join(ROOT, "deleglise-rivat_amd", "mdr_bench")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "--benchmark" in r.stdout and "--compare" in r.stdout
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    r = subprocess.run([exe, "1000"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3 and "mdr_init failed" in r.stderr

@@ -132,3 +132,21 @@ def test_largest_table_and_external_1e18():
         assert e.last_stats.U == U
     finally:
         e.close()
+
+
+def test_cli_modes(known):
+    exe = os.path.join(ROOT, "deleglise-rivat_amd", "mdr_bench")
+    r = subprocess.run([exe, str(10**12)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert f"M({10**12}) = {known[10**12]}" in r.stdout and "Timing Breakdown" in r.stdout
+    r = subprocess.run([exe, "--benchmark", "13", "0", "3"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = [l for l in r.stdout.splitlines() if l.startswith("| 10^")]
+    assert len(rows) == 13
+    for e, row in zip(range(1, 14), rows):
+        cells = [c.strip() for c in row.strip("|").split("|")]
+        assert int(cells[1]) == 10**e and int(cells[2]) == known[10**e], row
+    r = subprocess.run([exe, "--compare", "14"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = [l for l in r.stdout.splitlines() if l.startswith("| 10^")]
+    assert [int(l.strip("|").split("|")[1]) for l in rows] == [known[10**e] for e in range(10, 15)]
