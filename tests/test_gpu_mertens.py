@@ -150,3 +150,17 @@ def test_cli_modes(known):
     assert r.returncode == 0, r.stderr
     rows = [l for l in r.stdout.splitlines() if l.startswith("| 10^")]
     assert [int(l.strip("|").split("|")[1]) for l in rows] == [known[10**e] for e in range(10, 15)]
+
+
+def test_largest_int64_x_two_cutoffs_agree():
+    """X = 2^63 - 1, the top of the int64 range (no reference value exists):
+    the tuned cutoff (4e11) and a 3e11 cutoff must give the same M(X); the
+    64-bit S2 path (j >= 2^31) and the largest quotients are exercised."""
+    e = mdr_hip.Engine(0)
+    try:
+        X = 2**63 - 1
+        a = e.mertens(X, params=mdr_hip.PARAMS_TUNED, no_table_reuse=1)
+        b = e.mertens(X, params=mdr_hip.PARAMS_TUNED, tune_u=3 * 10**11, tune_ud=524160, no_table_reuse=1)
+        assert a == b
+    finally:
+        e.close()
