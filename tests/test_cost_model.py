@@ -118,3 +118,24 @@ def test_learned_partition_is_bit_exact(engine, known):
 def test_invalid_model_rejected_without_device():
     lib = mdr_hip.load()
     assert lib.mdr_cost_model_set(None, None, 0) == 1
+
+
+@pytest.mark.gpu
+def test_learned_partition_beats_the_term_partition(engine, known):
+    # What the trained network gains over the reference GPU engine's static
+    # unit of work (whole G evaluations per chunk): at 1e16 the heaviest term
+    # serialises one wave under the term partition.
+    import time
+    X = 10**16
+    def timed(partition):
+        o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=partition, no_table_reuse=1)
+        engine.mertens(X, o)
+        t0 = time.perf_counter()
+        r = engine.mertens(X, o)
+        return time.perf_counter() - t0, r, engine.last_stats.partition_used
+    t_terms, r_terms, _ = timed(mdr_hip.PARTITION_TERMS)
+    t_learned, r_learned, used = timed(mdr_hip.PARTITION_LEARNED)
+    assert r_terms == r_learned == known[X]
+    assert used == mdr_hip.PARTITION_LEARNED
+    print(f"terms {t_terms:.3f} s, learned {t_learned:.3f} s", flush=True)
+    assert t_terms > 1.5 * t_learned
