@@ -8,12 +8,13 @@ descent
   cx      the S1/S2 split factor (A = cx sqrt(y)),
   u_dense the dense int16 tier (everything above it in compressed lines),
   chunks  the quotient-phase chunk count of the dynamic queue (0 = automatic),
+  sieve   the sieve's segment length and workgroup shape,
 minimising the summed time of the two points.  Then it picks the batch cutoff
 factor on a log-uniform batch of X.  Every configuration must give the same
 M(X) (the result does not depend on the split); a mismatch aborts the run.
 
 Cache format (read by csrc/mdr_tune.cpp): '# ...' comments, one line per
-decade 'decade fx cx chunks u_dense', and one line 'batch fb'.
+decade 'decade fx cx chunks u_dense sieve_shape', and one line 'batch fb'.
 
 Usage: python autotune.py [--decades 8-18] [--reps 2] [--out PATH] [--quick]
 """
@@ -55,7 +56,8 @@ class Tuner:
         reps = self.reps if X < 10**17 else 1   # seconds-long calls: one run is stable
         o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1, sched=mdr_hip.SCHED_DYNAMIC,
                               partition=mdr_hip.PARTITION_LEARNED, tune_u=cutoff(X, cfg["fx"], cfg["cx"]),
-                              tune_ud=cfg["ud"], tune_cx=cfg["cx"], chunk=cfg["chunks"])
+                              tune_ud=cfg["ud"], tune_cx=cfg["cx"], chunk=cfg["chunks"],
+                              sieve_shape=cfg.get("sieve", 0) + 1)
         ts = []
         for _ in range(reps):
             t0 = time.perf_counter()
@@ -86,7 +88,7 @@ class Tuner:
 
 def tune_decade(T, d, waves, quick):
     xs = [10**d] + ([int(round(10 ** (d + 0.5)))] if d < 18 else [])
-    cfg = {"fx": 0.7, "cx": 2.0 if d >= 13 else 0.95 if d >= 11 else 0.7, "ud": 1048512, "chunks": 0}
+    cfg = {"fx": 0.7, "cx": 2.0 if d >= 13 else 0.95 if d >= 11 else 0.7, "ud": 1048512, "chunks": 0, "sieve": 0}
     log(f"decade {d}: X = {xs}")
     fxs = [0.35, 0.5, 0.7, 1.0, 1.4, 2.0] if not quick else [0.5, 0.7, 1.0]
     cxs = [0.7, 1.0, 1.35, 2.0, 2.8, 4.0] if not quick else [1.35, 2.0, 2.8]
@@ -98,6 +100,9 @@ def tune_decade(T, d, waves, quick):
     # second pass on fx around the choice (the best cutoff moves with cx and the tier)
     f = cfg["fx"]
     T.sweep(xs, cfg, "fx", sorted({round(f * m, 3) for m in (0.8, 1.0, 1.25)}))
+    # sieve segment length / workgroup (0 = 49152/512, 1 = 24576/256,
+    # 2 = 98304/1024, 3 = 49152/256): the table is the same, only its build time moves
+    T.sweep(xs, cfg, "sieve", [0, 1, 2, 3] if not quick else [0, 1])
     if d >= 14:
         t = T.sweep(xs, cfg, "chunks", chs)
     else:   # small calls: the automatic count (it adapts to the work) wins
@@ -155,14 +160,15 @@ def main():
             if f[0] == "batch":
                 fb = float(f[1])
             elif int(f[0]) not in range(a, b + 1):
-                kept[int(f[0])] = {"fx": float(f[1]), "cx": float(f[2]), "chunks": int(f[3]), "ud": int(f[4])}
+                kept[int(f[0])] = {"fx": float(f[1]), "cx": float(f[2]), "chunks": int(f[3]), "ud": int(f[4]),
+                                   "sieve": int(f[5]) if len(f) > 5 else 0}
     rows = sorted(list(kept.items()) + rows)
     if fb is None or not args.no_batch:
         fb, tb = tune_batch(E, 3, 4096, 10**9, 10**14, args.quick)
         report["batch"] = {"fb": fb, "ms": round(tb * 1000, 2)}
     lines = [f"# written by autotune.py on {time.strftime('%Y-%m-%d %H:%M:%S')}, {waves} resident quotient waves",
-             "# decade fx cx chunks u_dense"]
-    lines += [f"{d} {c['fx']} {c['cx']} {c['chunks']} {c['ud']}" for d, c in rows]
+             "# decade fx cx chunks u_dense sieve_shape"]
+    lines += [f"{d} {c['fx']} {c['cx']} {c['chunks']} {c['ud']} {c.get('sieve', 0)}" for d, c in rows]
     lines.append(f"batch {fb}")
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")

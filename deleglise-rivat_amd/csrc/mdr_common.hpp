@@ -21,15 +21,14 @@ namespace mdr {
 typedef long long i64;
 typedef unsigned long long u64;
 
-// Sieve segment: a multiple of 96 (so mu6 words align), of 64 (plane words)
-// and of 16 (aligned 16-byte stencil loads).
-constexpr int SEG = 49152;
-constexpr int SIEVE_THREADS = 512;
+// Sieve segment lengths are fixed per kernel instantiation (SIEVE_SHAPES in
+// mdr_kernels.hpp): multiples of 192 (compressed lines), 48 (packed mu words)
+// and 16 (aligned 16-byte stencil loads).
 // Stencil period lcm(16, 4, 9, 25, 7, 11): weights of 2,3,5,7,11 and the square
 // flags of 4, 9 and 25 repeat with it; 16 | period keeps phases 16-byte aligned.
 constexpr int STENCIL_PERIOD = 277200;
 // Values per compressed line; u_dense must be a multiple of it below U, and
-// SEG is a multiple of it, so lines never straddle segments.
+// every segment length is a multiple of it, so lines never straddle segments.
 constexpr int LINE_VALUES = 192;
 // Largest compressed tier (U - u_dense): line indices stay below 2^32.
 constexpr long long MAX_COMPRESSED_VALUES = (long long)LINE_VALUES * 4294967295LL;

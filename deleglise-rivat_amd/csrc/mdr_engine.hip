@@ -204,6 +204,8 @@ mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
         return fail(c, MDR_ERR_INVALID, "tuning override out of range");
     if (o.tune_u > 1000000000000LL) return fail(c, MDR_ERR_INVALID, "tune_u above 1e12");
     if (o.use_graph < 0 || o.use_graph > 1) return fail(c, MDR_ERR_INVALID, "use_graph must be 0 or 1");
+    if (o.sieve_shape < 0 || o.sieve_shape > N_SIEVE_SHAPES)
+        return fail(c, MDR_ERR_INVALID, "sieve_shape out of range");
     if (o.tune_batch != 0 && !(o.tune_batch >= 0.05 && o.tune_batch <= 20))
         return fail(c, MDR_ERR_INVALID, "tune_batch outside [0.05, 20]");
     return MDR_OK;
@@ -327,7 +329,7 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
     if (!c->stencil.p) {
         // value for n with n = x+1 (mod period): weights of 2,3,5,7,11 and the
         // square flags of 4, 9 and 25 (mertens_dr.hpp:784-793 uses 2..11, 4, 9).
-        std::vector<uint8_t> st((size_t)STENCIL_PERIOD + SEG);
+        std::vector<uint8_t> st((size_t)STENCIL_PERIOD + SEG_MAX);
         for (size_t x = 0; x < st.size(); ++x) {
             const unsigned v = (unsigned)(x % STENCIL_PERIOD) + 1;
             unsigned a = 0;
@@ -347,12 +349,13 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
 // A table request after validation: the sizes every kernel of the build sees.
 struct TableJob {
     bool build = false;   // false: the cached table already covers the request
+    int shape = 0;        // sieve workgroup shape (SIEVE_SHAPES)
     i64 U = 0, ud = 0, mu_lim = 0, NL = 0, nseg = 0, musegs = 0;
 };
 
 // Validates a table request and allocates what its build needs (host side:
 // everything that may allocate happens here, before any kernel is enqueued).
-mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, TableJob& j) {
+mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, int shape, TableJob& j) {
     if (U < 1) U = 1;
     if (ud > U) ud = U;
     if (ud < U && (ud % LINE_VALUES) != 0)
@@ -361,12 +364,15 @@ mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, Tabl
     if (mu_lim < 1) mu_lim = 1;
     if (U - ud > MAX_COMPRESSED_VALUES)
         return fail(c, MDR_ERR_RANGE, "table above the compressed-tier limit (U - u_dense > 192 * (2^32 - 1))");
+    if (shape < 0 || shape >= N_SIEVE_SHAPES) return fail(c, MDR_ERR_INVALID, "sieve shape out of range");
+    const i64 seg = SIEVE_SHAPES[shape].seg;
     j = TableJob{};
+    j.shape = shape;
     j.U = U;
     j.ud = ud;
     j.mu_lim = mu_lim;
-    j.nseg = (U + SEG - 1) / SEG;
-    j.musegs = (mu_lim + SEG - 1) / SEG;
+    j.nseg = (U + seg - 1) / seg;
+    j.musegs = (mu_lim + seg - 1) / seg;
     j.NL = U > ud ? (U - ud + LINE_VALUES - 1) / LINE_VALUES : 0;
     if (reuse && c->tab_valid && c->tU == U && c->tud == ud && c->tmu == mu_lim) return MDR_OK;
     j.build = true;
@@ -375,7 +381,7 @@ mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, Tabl
     if (s != MDR_OK) return s;
     HIPT(c->Md.ensure((size_t)(ud + 1) * 2), "alloc dense table");
     HIPT(c->cl.ensure((size_t)j.NL * sizeof(CLine)), "alloc compressed lines");
-    HIPT(c->mu6.ensure((size_t)j.musegs * (SEG / 48) * 4), "alloc mu");
+    HIPT(c->mu6.ensure((size_t)j.musegs * (seg / 48) * 4), "alloc mu");
     HIPT(c->segsum.ensure((size_t)j.nseg * 8), "alloc segment sums");
     HIPT(c->scan_tmp.ensure((size_t)scan_tmp_len(j.nseg) * 8), "alloc scan");
     // the quotient kernels read the table through these from now on; the
@@ -409,20 +415,20 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     a.cl = c->cl.as<CLine>();
     a.mu6 = c->mu6.as<uint32_t>();
     a.segsum = c->segsum.as<i64>();
-    // a dispatch holds fewer than 2^32 work-items: at most 2^22 blocks of 512
-    const i64 maxgrid = 1 << 22;
+    // a dispatch holds fewer than 2^32 work-items: at most 2^31 threads per launch
+    const i64 maxgrid = ((i64)1 << 31) / SIEVE_SHAPES[j.shape].threads;
     for (i64 s0 = 0; s0 < j.nseg; s0 += maxgrid) {
         a.seg_first = s0;
         const i64 nb = std::min(maxgrid, j.nseg - s0);
-        hipLaunchKernelGGL(k_sieve, dim3((unsigned)nb), dim3(SIEVE_THREADS), 0, c->st, a);
+        HIPT(launch_sieve(j.shape, (unsigned)nb, c->st, a), "launch sieve");
     }
-    HIPT(hipGetLastError(), "launch sieve");
     HIPT(scan_exclusive_i64(c->segsum.as<i64>(), c->segsum.as<i64>(), j.nseg, c->scan_tmp.as<i64>(),
                             scan_tmp_len(j.nseg), nullptr, c->st),
          "scan segments");
     FixupArgs f;
     f.ud = j.ud;
     f.L = j.NL;
+    f.seg_shift = SIEVE_SHAPES[j.shape].shift;
     f.Md = c->Md.as<int16_t>();
     f.cl = c->cl.as<CLine>();
     f.segoff = c->segsum.as<i64>();
@@ -434,13 +440,14 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
 }
 
 double table_bytes(const TableJob& j) {
-    return 2.0 * (double)(j.ud + 1) + (double)sizeof(CLine) * (double)j.NL + (double)j.musegs * (SEG / 12);
+    return 2.0 * (double)(j.ud + 1) + (double)sizeof(CLine) * (double)j.NL +
+           (double)j.musegs * (SIEVE_SHAPES[j.shape].seg / 12);
 }
 
 // Builds (or reuses) the table and waits for it.
-mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, bool reuse = true) {
+mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, bool reuse = true, int shape = 0) {
     TableJob j;
-    mdr_status s = prepare_table(c, U, ud, mu_lim, reuse, j);
+    mdr_status s = prepare_table(c, U, ud, mu_lim, reuse, shape, j);
     if (s != MDR_OK) return s;
     if (!j.build) {
         if (stats) stats->table_reused = 1;
@@ -612,7 +619,15 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     BatchLaunch b;
     b.nx = nx;
     if (need_table) {
-        mdr_status s = prepare_table(c, U, ud, mu_lim, !o.no_table_reuse, b.tj);
+        // sieve workgroup shape: explicit, else the tuned one of the largest X
+        int shape = o.sieve_shape > 0 ? o.sieve_shape - 1 : 0;
+        if (o.sieve_shape == 0 && o.params == MDR_PARAMS_TUNED) {
+            i64 xmax = 1;
+            for (const XPlan& p : plans)
+                if (p.kind != 0) xmax = std::max(xmax, p.X);
+            shape = tune_sieve_shape(xmax);
+        }
+        mdr_status s = prepare_table(c, U, ud, mu_lim, !o.no_table_reuse, shape, b.tj);
         if (s != MDR_OK) return s;
         if (stats && !b.tj.build) stats->table_reused = 1;
     }

@@ -26,6 +26,7 @@ struct SieveArgs {
 
 struct FixupArgs {
     i64 ud, L;
+    int seg_shift;             // segment length 3 * 2^seg_shift
     int16_t* Md;
     CLine* cl;
     const i64* segoff;
@@ -73,7 +74,16 @@ struct QuotArgs {
     int stamp_stride;              // log slots whose hash is 0 mod stride
 };
 
-__global__ void k_sieve(SieveArgs a);
+// Sieve workgroup shapes (segment length, threads): the table they build is
+// the same; the autotuner picks one per decade of X.  Segment lengths are
+// 3 * 2^shift, multiples of 192 (lines), 48 (mu words) and 16.
+struct SieveShape {
+    int seg, threads, shift;
+};
+constexpr SieveShape SIEVE_SHAPES[] = {{49152, 512, 14}, {24576, 256, 13}, {98304, 1024, 15}, {49152, 256, 14}};
+constexpr int N_SIEVE_SHAPES = 4;
+constexpr int SEG_MAX = 98304;
+hipError_t launch_sieve(int shape, unsigned blocks, hipStream_t st, const SieveArgs& a);
 __global__ void k_table_fixup(FixupArgs a);
 __global__ void k_slot_len(PartArgs a);
 __global__ void k_cuts(PartArgs a);

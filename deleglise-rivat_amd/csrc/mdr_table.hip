@@ -17,7 +17,6 @@ namespace mdr {
 
 namespace {
 
-constexpr int GROUPS = SEG / 64;          // 768 groups of 64 values
 
 __device__ __forceinline__ uint32_t mod_u64(u64 x, uint32_t p, double rp) {
     // x < 2^52; estimate within 1 of floor(x/p), then fix.
@@ -37,9 +36,11 @@ __device__ __forceinline__ int8_t finalize(uint8_t a, u64 n) {
 
 }  // namespace
 
-__global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
-    __shared__ __align__(16) uint32_t acc32[SEG / 4];
-    __shared__ __align__(16) uint32_t nib32[SEG / 16];   // per dword: nonzero nibble | negative nibble << 4
+template <int SEGT, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
+    constexpr int GROUPS = SEGT / 64;      // 64-value words per segment
+    __shared__ __align__(16) uint32_t acc32[SEGT / 4];
+    __shared__ __align__(16) uint32_t nib32[SEGT / 16];   // per dword: nonzero nibble | negative nibble << 4
     __shared__ int32_t gsum[GROUPS];
     __shared__ int32_t gpre[GROUPS];
     __shared__ uint32_t small_off[64];
@@ -50,8 +51,8 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const u64 base = (u64)s * SEG;
-    const u64 lastn = (u64)(base + SEG <= (u64)a.U ? base + SEG : (u64)a.U);
+    const u64 base = (u64)s * SEGT;
+    const u64 lastn = (u64)(base + SEGT <= (u64)a.U ? base + SEGT : (u64)a.U);
     const int len = (int)(lastn - base);
 
     // 1. periodic stencil: weights of 2,3,5,7,11 and flags of 4, 9, 25.
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
         const uint32_t phase = (uint32_t)(base % STENCIL_PERIOD);
         const uint4* src = reinterpret_cast<const uint4*>(a.stencil + phase);
         uint4* dst = reinterpret_cast<uint4*>(acc32);
-        for (int i = tid; i < SEG / 16; i += SIEVE_THREADS) dst[i] = src[i];
+        for (int i = tid; i < SEGT / 16; i += THREADS) dst[i] = src[i];
     }
     // offsets of the block-cooperative primes
     const u64 first = base + 1;
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     // 2. square flags for 7 <= p < 256: wave w takes primes i = w (mod 8), its
     //    lanes the consecutive multiples (a block-wide loop per prime costs
     //    more in per-thread set-up than the few hits it has).
-    constexpr int NWAVES = SIEVE_THREADS / 64;
+    constexpr int NWAVES = THREADS / 64;
     for (int i = wave; i < a.n_sq_small; i += NWAVES) {
         const uint32_t p = a.sqprimes[i];
         const uint32_t p2 = p * p;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
         const uint32_t p = a.wprimes[i3];
         if (p >= 64) break;
         const uint32_t w = a.wweights[i3];
-        for (uint32_t x = small_off[i3] + (uint32_t)tid * p; x < (uint32_t)len; x += SIEVE_THREADS * p)
+        for (uint32_t x = small_off[i3] + (uint32_t)tid * p; x < (uint32_t)len; x += THREADS * p)
             atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
     }
     for (int i = i3 + wave; i < a.n_small; i += NWAVES) {
@@ -113,11 +114,11 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     constexpr int UNR = 4;
     const double dfirst = (double)first;
     const double dlen = (double)len;
-    for (int i0 = a.n_small + tid; i0 < a.n_w; i0 += UNR * SIEVE_THREADS) {
+    for (int i0 = a.n_small + tid; i0 < a.n_w; i0 += UNR * THREADS) {
         uint32_t pp[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const int i = i0 + u * SIEVE_THREADS;
+            const int i = i0 + u * THREADS;
             pp[u] = i < a.n_w ? a.wprimes[i] : 0xffffffffu;
         }
         if ((u64)pp[0] * pp[0] > lastn) break;
@@ -131,8 +132,8 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
             // one correction each
             double r2 = __builtin_fma(-__builtin_rint(dfirst * (rp * rp)), dp2, dfirst);
             r2 += r2 < 0.0 ? dp2 : 0.0;
-            const double xs = r2 != 0.0 ? dp2 - r2 : 0.0;   // square hit offset
-            if (xs < dlen) {
+            // square hits (more than one when p^2 is below the segment length)
+            for (double xs = r2 != 0.0 ? dp2 - r2 : 0.0; xs < dlen; xs += dp2) {
                 const uint32_t x = (uint32_t)xs;
                 atomicOr(&acc32[x >> 2], 0x80u << (8 * (x & 3)));
             }
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     {
         const unsigned tseg = 63u - (unsigned)__clzll(first);
         const u64 db = (2ull << tseg) - first;                  // b - first >= 1
-        const uint32_t gb = db >= (u64)SEG ? 0xffffffffu : (uint32_t)(db >> 4);
+        const uint32_t gb = db >= (u64)SEGT ? 0xffffffffu : (uint32_t)(db >> 4);
         const bool straddle = (db & 15) != 0;
         const uint32_t gend = (uint32_t)len >> 4;               // groups wholly <= U
         const bool general = first < (1ull << 17);
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
             const uint32_t ngn = ((((mu >> 7) & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
             return nzn | (ngn << 4);
         };
-        for (uint32_t g = tid; g < SEG / 16; g += SIEVE_THREADS) {
+        for (uint32_t g = tid; g < SEGT / 16; g += THREADS) {
             uint4 v = acc4[g];
             if (!general && g < gend && !(straddle && g == gb)) {
                 const uint32_t add = g < gb ? add_lo : add_hi;
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     __syncthreads();
 
     // 6. one lane per 64-value word: bit planes from 16 nibble bytes, word sums
-    for (int k = tid; k < GROUPS; k += SIEVE_THREADS) {
+    for (int k = tid; k < GROUPS; k += THREADS) {
         const uint4 b = reinterpret_cast<const uint4*>(nib32)[k];
         auto lo16 = [](uint32_t v) {
             v &= 0x0f0f0f0fu;
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
 
     // 7. checkpoints of compressed words (segment-local; the fix-up adds the
     //    segment offset) and dense prefix values
-    for (int k = tid; k < GROUPS; k += SIEVE_THREADS) {
+    for (int k = tid; k < GROUPS; k += THREADS) {
         const u64 n0 = first + (u64)k * 64;
         if (n0 > (u64)a.ud && n0 <= (u64)a.U) {
             const u64 w = (n0 - (u64)a.ud - 1) >> 6;
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
         }
     }
     if (first <= (u64)a.ud) {
-        for (int i = tid; i < SEG / 4; i += SIEVE_THREADS) {
+        for (int i = tid; i < SEGT / 4; i += THREADS) {
             const uint32_t x = acc32[i];
             const int b0 = (int8_t)x, b1 = (int8_t)(x >> 8), b2 = (int8_t)(x >> 16), b3 = (int8_t)(x >> 24);
             const int s4 = b0 + b1 + b2 + b3;
@@ -286,8 +287,8 @@ __global__ __launch_bounds__(SIEVE_THREADS) void k_sieve(SieveArgs a) {
     // 8. packed mu for j coprime to 6 (t = j/3; 16 two's-complement 2-bit
     //    codes per word)
     if (s < a.mu_segs) {
-        const i64 w0 = s * (SEG / 48);
-        for (int wi = tid; wi < SEG / 48; wi += SIEVE_THREADS) {
+        const i64 w0 = s * (SEGT / 48);
+        for (int wi = tid; wi < SEGT / 48; wi += THREADS) {
             uint32_t word = 0;
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
@@ -311,16 +312,27 @@ __global__ void k_table_fixup(FixupArgs a) {
     if (i0 == 0) a.Md[0] = 0;
     for (i64 i = i0; i < a.ud; i += stride) {
         const i64 n = i + 1;
-        const i64 s = (n - 1) / SEG;
+        const i64 s = ((n - 1) >> a.seg_shift) / 3;
         a.Md[n] = (int16_t)(a.Md[n] + (int32_t)a.segoff[s]);
     }
     for (i64 i = i0; i < a.L; i += stride) {   // line i: its three checkpoints
-        const int32_t add = (int32_t)a.segoff[(a.ud + (i64)LINE_VALUES * i) / SEG];
+        const int32_t add = (int32_t)a.segoff[((a.ud + (i64)LINE_VALUES * i) >> a.seg_shift) / 3];
         int32_t* ck = a.cl[i].ck;
         ck[0] += add;
         ck[1] += add;
         ck[2] += add;
     }
+}
+
+hipError_t launch_sieve(int shape, unsigned blocks, hipStream_t st, const SieveArgs& a) {
+    switch (shape) {
+        case 0: hipLaunchKernelGGL((k_sieve_t<49152, 512>), dim3(blocks), dim3(512), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_sieve_t<24576, 256>), dim3(blocks), dim3(256), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_sieve_t<98304, 1024>), dim3(blocks), dim3(1024), 0, st, a); break;
+        case 3: hipLaunchKernelGGL((k_sieve_t<49152, 256>), dim3(blocks), dim3(256), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace mdr

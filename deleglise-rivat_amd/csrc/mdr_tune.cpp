@@ -2,7 +2,7 @@
 //
 // The cache is a text file written by the project's autotuner
 // (deleglise-rivat_amd/autotune.py): one line per decade of X,
-//     decade  fx  cx  chunks  u_dense
+//     decade  fx  cx  chunks  u_dense  [sieve shape]
 // and optionally one line "batch fb" (the batch cutoff factor),
 // where u = fx * (X / ln ln X)^(2/3) (the reference's cutoff form,
 // mertens_dr.hpp:1401-1419, with a GPU-specific factor), chunks is the
@@ -31,6 +31,7 @@ struct Entry {
     double fx, cx;
     i64 chunk;
     i64 ud;
+    int sieve;   // sieve workgroup shape (0 = the default)
 };
 
 std::once_flag g_once;
@@ -65,10 +66,12 @@ void load() {
             if (fb >= 0.05 && fb <= 20) g_batch_factor = fb;
             continue;
         }
-        const int k = std::sscanf(line, "%d %lf %lf %lld %lld", &e.decade, &e.fx, &e.cx, &ch, &ud);
+        int sv = 0;
+        const int k = std::sscanf(line, "%d %lf %lf %lld %lld %d", &e.decade, &e.fx, &e.cx, &ch, &ud, &sv);
         if (k >= 4 && e.fx > 0.05 && e.fx < 100 && e.cx > 0.1 && e.cx < 5.0 && ch >= 0) {
             e.chunk = ch;
-            e.ud = k == 5 && ud >= 0 ? ud : 0;
+            e.ud = k >= 5 && ud >= 0 ? ud : 0;
+            e.sieve = k >= 6 && sv >= 0 && sv < 4 ? sv : 0;
             g_entries.push_back(e);
         }
     }
@@ -142,6 +145,14 @@ i64 tune_chunks(i64 X) {
 }
 
 i64 tune_chunk() { return tune_chunks(1); }
+
+int tune_sieve_shape(i64 X) {
+    std::call_once(g_once, load);
+    const int dec = (int)std::floor(std::log10((double)(X > 1 ? X : 1)));
+    for (const Entry& e : g_entries)
+        if (e.decade == dec) return e.sieve;
+    return 0;
+}
 
 void tune_set_device_memory(size_t bytes) {
     // compressed lines take 1/3 byte per value: 45 % of the memory at most

@@ -20,6 +20,7 @@ i64 tune_chunk();
 i64 tune_chunks(i64 X);   // tuned quotient-phase chunk count for X (0 = automatic)
 // shared cutoff of a batch whose X values sum to xsum (factor 0 = tuned factor)
 i64 tune_batch_u(double xsum, double factor = 0);
+int tune_sieve_shape(i64 X);   // tuned sieve workgroup shape for X (index into SIEVE_SHAPES)
 // Lowers the largest tuned cutoff to what a device of this memory holds.
 void tune_set_device_memory(size_t bytes);
 

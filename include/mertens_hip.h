@@ -87,6 +87,9 @@ typedef struct {
                                  (instantiated once per launch sequence, updated in place) */
     double tune_batch;        /* batch cutoff factor for MDR_PARAMS_TUNED batches
                                  (0 = from the tuning cache; accepted 0.05..20) */
+    int sieve_shape;          /* sieve segment length / workgroup: 0 = automatic (tuned with
+                                 MDR_PARAMS_TUNED), 1 = 49152/512, 2 = 24576/256,
+                                 3 = 98304/1024, 4 = 49152/256; the table is the same */
 } mdr_opts;
 
 typedef struct {

@@ -93,3 +93,26 @@ def test_compressed_tier_limit_is_an_error(engine):
     with pytest.raises(mdr_hip.MdrError, match="RANGE|compressed-tier"):
         engine.mertens(10**18, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=9 * 10**11,
                                                  tune_ud=ud, no_table_reuse=1))
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3, 4])
+def test_sieve_shapes_build_the_same_table(engine, mu_ref, pre_ref, known, shape):
+    # every sieve workgroup shape (segment length / threads) builds the same
+    # table: a call leaves its table in the context, whose M and mu windows
+    # are then checked against Eratosthenes, and M(X) against the known values
+    for X in (10**12, 10**16):
+        assert engine.mertens(X, params=mdr_hip.PARAMS_TUNED, sieve_shape=shape, no_table_reuse=1) == known[X]
+    st = engine.last_stats
+    assert st.U > LIM
+    M = engine.read_M(0, LIM + 1).astype(np.int64)
+    assert np.array_equal(M, pre_ref[:LIM + 1])
+    ml = min(st.mu_limit, LIM)
+    assert np.array_equal(engine.read_mu(0, ml + 1), mu_ref[:ml + 1])
+    hi = st.U - 40000   # the top of the table: M differences against trial division
+    Mh = engine.read_M(hi - 1, 40001).astype(np.int64)
+    assert np.array_equal(np.diff(Mh), segmented_mobius(hi, hi + 39999))
+
+
+def test_sieve_shape_out_of_range(engine):
+    with pytest.raises(mdr_hip.MdrError):
+        engine.mertens(10**12, sieve_shape=5)
