@@ -92,7 +92,7 @@ def roofline(stats):
             "valu_issue_frac": issue.get("valu_issue_frac"), "l2_hit_rate": issue.get("l2_hit_rate"),
             "note": "per-unit bytes count every compressed lookup as a 20-byte HBM fetch; L2/MALL serve part of "
                     "them (traffic, from FETCH/WRITE_SIZE, is below the algorithmic bytes), so frac can exceed 1; "
-                    "the binding resource is VALU issue (valu_issue_frac, 4 cycles per wave64 op, a lower bound)"}
+                    "the binding resource is VALU issue (valu_issue_frac: SQ_INSTS_VALU at the nominal 4 cycles per wave64 instruction)"}
 
 
 def main():
