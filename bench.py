@@ -89,10 +89,12 @@ def roofline(stats):
     return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_quot",
             "algorithmic_bytes": byts, "kernel_ms": round(ms, 3), "launches": len(stats),
-            "valu_issue_frac": issue.get("valu_issue_frac"), "l2_hit_rate": issue.get("l2_hit_rate"),
+            "valu_issue_frac": issue.get("valu_issue_frac"), "valu_busy": issue.get("valu_busy"),
+            "l2_hit_rate": issue.get("l2_hit_rate"),
             "note": "per-unit bytes count every compressed lookup as a 20-byte HBM fetch; L2/MALL serve part of "
                     "them (traffic, from FETCH/WRITE_SIZE, is below the algorithmic bytes), so frac can exceed 1; "
-                    "the binding resource is VALU issue (valu_issue_frac: SQ_INSTS_VALU at the nominal 4 cycles per wave64 instruction)"}
+                    "the binding resource is VALU issue: valu_busy (SQ_ACTIVE_INST_VALU over the SIMDs' cycles) is the measured "
+                    "fraction of SIMD cycles spent issuing VALU work, valu_issue_frac the same from SQ_INSTS_VALU at 4 cycles each"}
 
 
 def main():
