@@ -81,6 +81,13 @@ struct TableParam {
     const int16_t* Md;
     const CLine* cl;
     const uint32_t* mu6;
+    // Quotient pass over a windowed table: 0 = the whole table is resident;
+    // 1 = the resident part (every position, compressed lookups only for
+    // q <= qhi); 2 = one window of the compressed tier (only the compressed
+    // lookups with qlo <= q <= qhi, from lines that start at qlo = ud + 1).
+    // Passes 0 and 1 have qlo = ud + 1.
+    int pass;
+    i64 qlo, qhi;
 };
 
 }  // namespace mdr

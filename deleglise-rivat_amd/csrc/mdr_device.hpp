@@ -240,7 +240,8 @@ __device__ __forceinline__ int frame_nseg(const Frame& f) {
 
 // S1 run of stride s over [lo, hi] split at the dense/compressed boundary:
 // n <= split gives q > ud (compressed), n > split gives q <= ud (dense).
-__device__ __forceinline__ Seg s1_part(i64 lo, i64 hi, i64 s, i64 split, bool comp, int sign) {
+// Compressed positions also need n >= nmin (q at most the pass's qhi).
+__device__ __forceinline__ Seg s1_part(i64 lo, i64 hi, i64 s, i64 split, bool comp, int sign, i64 nmin) {
     Seg r{};
     r.type = comp ? SEG_S1_COMP : SEG_S1_DENSE;
     r.sign = sign;
@@ -248,6 +249,10 @@ __device__ __forceinline__ Seg s1_part(i64 lo, i64 hi, i64 s, i64 split, bool co
     i64 a = lo, b = hi;
     if (comp) {
         if (b > split) b = split;
+        if (a < nmin) {
+            a = nmin;
+            if (s == 2 && ((a - lo) & 1)) ++a;
+        }
     } else if (a <= split) {
         a = split + 1;
         if (s == 2 && ((a - lo) & 1)) ++a;
@@ -272,9 +277,15 @@ __device__ __forceinline__ Seg s2_part(i64 lo, i64 hi, int piece) {
 
 __device__ inline Seg frame_seg(const Frame& f, const XParam& xp, const TableParam& tp, int idx) {
     const i64 y = f.y;
-    const i64 split = y / (tp.ud + 1);
+    // compressed lookups of this pass: qlo <= q <= qhi, i.e. nmin <= n <= split;
+    // a window pass (2) has no dense lookups, S2 pieces or constants
+    const i64 split = y / tp.qlo;
+    const i64 nmin = y / (tp.qhi + 1) + 1;
+    const bool win = tp.pass == 2;
+    if (win && (idx & 1) && idx < (f.comb2 ? 4 : 2)) return Seg{};   // dense parts
+    if (win && idx >= (f.comb2 ? 4 : 2)) return Seg{};               // S2 pieces
     if (!f.comb2) {
-        if (idx < 2) return s1_part(y / xp.u + 1, f.ky, 1, split, idx == 0, -1);
+        if (idx < 2) return s1_part(y / xp.u + 1, f.ky, 1, split, idx == 0, -1, nmin);
         const i64 A = f.A;
         const i64 cut[5] = {0, A / 6, A / 3, A / 2, A};
         return s2_part(cut[idx - 2] + 1, cut[idx - 1], 9 + (idx - 2));
@@ -282,7 +293,7 @@ __device__ inline Seg frame_seg(const Frame& f, const XParam& xp, const TablePar
     if (idx < 2) {
         i64 lo = y / xp.u + 1;
         if (!(lo & 1)) ++lo;
-        return s1_part(lo, f.ky, 2, split, idx == 0, -1);
+        return s1_part(lo, f.ky, 2, split, idx == 0, -1, nmin);
     }
     if (idx < 4) {
         const i64 two = 2 * f.ky2;
@@ -292,7 +303,7 @@ __device__ inline Seg frame_seg(const Frame& f, const XParam& xp, const TablePar
         else if (two < f.ky) { lo = two + 1; hi = f.ky; sign = -1; }
         else return Seg{};
         if (lo & 1) ++lo;
-        return s1_part(lo, hi, 2, split, idx == 2, sign);
+        return s1_part(lo, hi, 2, split, idx == 2, sign, nmin);
     }
     const i64 A = f.A, B = f.B;
     const i64 cut[9] = {0, B / 6, A / 6, B / 3, A / 3, B / 2, A / 2, B, A};

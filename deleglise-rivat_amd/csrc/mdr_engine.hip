@@ -56,6 +56,12 @@ struct DevBuf {
         p = nullptr;
         cap = 0;
     }
+    // ensure, after giving back an allocation far larger than needed (a big
+    // table must not hold the memory the next one needs)
+    hipError_t fit(size_t bytes) {
+        if (p && cap > 2 * bytes && cap - bytes > ((size_t)1 << 30)) release();
+        return ensure(bytes);
+    }
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
@@ -86,6 +92,9 @@ struct mdr_ctx {
     i64 primes_root = -1;
     int n_w = 0, n_small = 0, n_sq = 0, n_sq_small = 0;
     DevBuf stencil, wprimes, wrcp, wweights, sqprimes;
+    // windows of the compressed tier (windowed tables): lines, segment sums,
+    // scan scratch, [carry M(window start - 1), window total]
+    DevBuf wcl, wseg, wscan, wcarry;
     // work
     DevBuf xparam, slot_len, slot_cost, bounds, out128, ctr, stamps, comp_out, lookup;
     std::vector<StampRec> last_stamps;
@@ -105,6 +114,9 @@ struct mdr_ctx {
         t.Md = Md.as<int16_t>();
         t.cl = cl.as<CLine>();
         t.mu6 = mu6.as<uint32_t>();
+        t.pass = 0;
+        t.qlo = tud + 1;
+        t.qhi = tU;
         return t;
     }
 };
@@ -186,6 +198,7 @@ struct XPlan {
     bool streaming = false;
     i64 u = 0, ud = 0, mu_lim = 0, N = 0;
     double cx = 0;
+    i64 window = 0;    // tuned plans whose cutoff exceeds one table: window size
 };
 
 mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
@@ -208,6 +221,7 @@ mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
         return fail(c, MDR_ERR_INVALID, "sieve_shape out of range");
     if (o.tune_batch != 0 && !(o.tune_batch >= 0.05 && o.tune_batch <= 20))
         return fail(c, MDR_ERR_INVALID, "tune_batch outside [0.05, 20]");
+    if (o.window < 0) return fail(c, MDR_ERR_INVALID, "window must be >= 0");
     return MDR_OK;
 }
 
@@ -237,6 +251,7 @@ XPlan plan_x(i64 X, const mdr_opts& o) {
                                               ((i64)(t.cx * std::sqrt((double)X)) + 1000 + LINE_VALUES) / LINE_VALUES * LINE_VALUES);
         if (o.tune_ud > 0) t.ud = std::min(t.u, (o.tune_ud / LINE_VALUES) * LINE_VALUES);
         p.cx = t.cx;
+        if (o.tune_u == 0) p.window = t.window;
         const i64 A2 = (i64)(p.cx * std::sqrt((double)X)) + 1000;
         p.u = t.u;
         p.ud = std::min(p.u, t.ud);
@@ -379,8 +394,8 @@ mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, int 
     c->tab_valid = false;
     mdr_status s = ensure_primes(c, U);
     if (s != MDR_OK) return s;
-    HIPT(c->Md.ensure((size_t)(ud + 1) * 2), "alloc dense table");
-    HIPT(c->cl.ensure((size_t)j.NL * sizeof(CLine)), "alloc compressed lines");
+    HIPT(c->Md.fit((size_t)(ud + 1) * 2), "alloc dense table");
+    HIPT(c->cl.fit((size_t)j.NL * sizeof(CLine)), "alloc compressed lines");
     HIPT(c->mu6.ensure((size_t)j.musegs * (seg / 48) * 4), "alloc mu");
     HIPT(c->segsum.ensure((size_t)j.nseg * 8), "alloc segment sums");
     HIPT(c->scan_tmp.ensure((size_t)scan_tmp_len(j.nseg) * 8), "alloc scan");
@@ -415,6 +430,7 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     a.cl = c->cl.as<CLine>();
     a.mu6 = c->mu6.as<uint32_t>();
     a.segsum = c->segsum.as<i64>();
+    a.seg0 = 0;
     // a dispatch holds fewer than 2^32 work-items: at most 2^31 threads per launch
     const i64 maxgrid = ((i64)1 << 31) / SIEVE_SHAPES[j.shape].threads;
     for (i64 s0 = 0; s0 < j.nseg; s0 += maxgrid) {
@@ -428,10 +444,13 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     FixupArgs f;
     f.ud = j.ud;
     f.L = j.NL;
+    f.nd = j.ud;
     f.seg_shift = SIEVE_SHAPES[j.shape].shift;
     f.Md = c->Md.as<int16_t>();
     f.cl = c->cl.as<CLine>();
     f.segoff = c->segsum.as<i64>();
+    f.seg0 = 0;
+    f.carry = nullptr;
     const i64 nfix = std::max<i64>(1, std::max(j.ud, j.NL));
     const i64 fix_blocks = std::min<i64>((nfix + 255) / 256, 1 << 20);   // grid-stride kernel
     hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)fix_blocks), dim3(256), 0, c->st, f);
@@ -476,6 +495,12 @@ __global__ void k_lookup(TableParam tp, const i64* qs, const int* xi, int n, u64
     out128[2 * xi[i] + 1] = (u64)(v >> 63);
 }
 
+// carry[0] = M(q) from the resident table (add == nullptr), else carry[0] += add[0]:
+// the prefix a window's checkpoints start from.
+__global__ void k_window_carry(TableParam tp, i64 q, i64* carry, const i64* add) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) carry[0] = add ? carry[0] + add[0] : (i64)get_M(tp, q);
+}
+
 // ---- one call: table, lookups, partition, quotient phase -------------------------------
 // The call is prepared on the host (sizes, allocations, uploads), then enqueued
 // as kernel launches only, either straight onto the engine's stream or
@@ -490,6 +515,29 @@ struct BatchLaunch {
     PartArgs pa{};
     QuotArgs qa{};
 };
+
+// Partition and quotient phase of one pass: slot lengths (and predicted
+// costs), their scans, the chunk cuts and the persistent quotient kernel.
+mdr_status enqueue_quotient(mdr_ctx* c, const BatchLaunch& b, const PartArgs& pa, const QuotArgs& qa, bool timed) {
+    unsigned long long* ctr = c->ctr.as<unsigned long long>();
+    if (timed) HIPT(hipEventRecord(c->ev[2], c->st), "event");
+    hipLaunchKernelGGL(k_slot_len, dim3((unsigned)((b.slots + 255) / 256)), dim3(256), 0, c->st, pa);
+    HIPT(hipGetLastError(), "launch slot lengths");
+    HIPT(scan_exclusive_i64(pa.len, pa.len, b.slots, c->scan_tmp.as<i64>(), scan_tmp_len(b.slots),
+                            (i64*)(ctr + 4), c->st),
+         "scan positions");
+    if (b.learned)
+        HIPT(scan_exclusive_i64(pa.cost, pa.cost, b.slots, c->scan_tmp.as<i64>(), scan_tmp_len(b.slots),
+                                (i64*)(ctr + 5), c->st),
+             "scan costs");
+    hipLaunchKernelGGL(k_cuts, dim3((unsigned)((b.P + 1 + 255) / 256)), dim3(256), 0, c->st, pa);
+    HIPT(hipGetLastError(), "launch cuts");
+    if (timed) HIPT(hipEventRecord(c->ev[3], c->st), "event");
+    const unsigned grid = (unsigned)(c->ncu * c->quot_blocks_per_cu);
+    hipLaunchKernelGGL(k_quot, dim3(grid), dim3(256), 0, c->st, qa);
+    HIPT(hipGetLastError(), "launch quotient kernel");
+    return MDR_OK;
+}
 
 // Kernel launches of one call (no allocation, no memset, no copy, no host
 // wait), so the same code runs eagerly and under stream capture.
@@ -512,24 +560,108 @@ mdr_status enqueue_batch(mdr_ctx* c, const BatchLaunch& b, bool timed) {
         HIPT(hipGetLastError(), "launch lookup");
     }
     if (b.nxs) {
-        if (timed) HIPT(hipEventRecord(c->ev[2], c->st), "event");
-        hipLaunchKernelGGL(k_slot_len, dim3((unsigned)((b.slots + 255) / 256)), dim3(256), 0, c->st, b.pa);
-        HIPT(hipGetLastError(), "launch slot lengths");
-        HIPT(scan_exclusive_i64(b.pa.len, b.pa.len, b.slots, c->scan_tmp.as<i64>(), scan_tmp_len(b.slots),
-                                (i64*)(ctr + 4), c->st),
-             "scan positions");
-        if (b.learned)
-            HIPT(scan_exclusive_i64(b.pa.cost, b.pa.cost, b.slots, c->scan_tmp.as<i64>(), scan_tmp_len(b.slots),
-                                    (i64*)(ctr + 5), c->st),
-                 "scan costs");
-        hipLaunchKernelGGL(k_cuts, dim3((unsigned)((b.P + 1 + 255) / 256)), dim3(256), 0, c->st, b.pa);
-        HIPT(hipGetLastError(), "launch cuts");
-        if (timed) HIPT(hipEventRecord(c->ev[3], c->st), "event");
-        const unsigned grid = (unsigned)(c->ncu * c->quot_blocks_per_cu);
-        hipLaunchKernelGGL(k_quot, dim3(grid), dim3(256), 0, c->st, b.qa);
-        HIPT(hipGetLastError(), "launch quotient kernel");
+        mdr_status s = enqueue_quotient(c, b, b.pa, b.qa, timed);
+        if (s != MDR_OK) return s;
     }
     if (timed) HIPT(hipEventRecord(c->ev[4], c->st), "event");
+    return MDR_OK;
+}
+
+// ---- windowed tables --------------------------------------------------------------------
+// The resident table covers 1..U0 (mu, the dense tier, every M the constants
+// and lookups read, and the compressed tier up to U0); the compressed tier of
+// U0 < q <= U streams through windows [L, R] of at most wvals values (whole
+// sieve segments).  Each window is sieved into its own lines, its checkpoints
+// continue from the carried M(L - 1), and a quotient pass sums the S1
+// lookups whose quotient falls in it (the positions do not depend on the
+// window, so the sum over the passes equals the one-table sum).
+// Reference: compute_mertens_windowed (mertens_dr_windowed.hpp:49-407), which
+// streams int16 M windows over 1..u built from a retained mu sieve.
+struct WindowPlan {
+    i64 U0 = 0, U = 0, wvals = 0;
+    int shape = 0;
+    i64 count() const { return wvals > 0 && U > U0 ? (U - U0 + wvals - 1) / wvals : 0; }
+};
+
+mdr_status prepare_windows(mdr_ctx* c, const WindowPlan& w) {
+    mdr_status st = ensure_primes(c, w.U);   // the windows sieve up to sqrt(U)
+    if (st != MDR_OK) return st;
+    const i64 seg = SIEVE_SHAPES[w.shape].seg;
+    const i64 nl = (w.wvals + LINE_VALUES - 1) / LINE_VALUES;
+    const i64 ns = w.wvals / seg + 1;
+    HIPT(c->wcl.fit((size_t)nl * sizeof(CLine)), "alloc window lines");
+    HIPT(c->wseg.ensure((size_t)ns * 8), "alloc window sums");
+    HIPT(c->wscan.ensure((size_t)scan_tmp_len(ns) * 8), "alloc window scan");
+    HIPT(c->wcarry.ensure(16), "alloc window carry");
+    return MDR_OK;
+}
+
+TableParam window_param(mdr_ctx* c, const TableParam& res, i64 L, i64 R) {
+    TableParam t = res;   // mu, the dense tier and mu_lim stay the resident ones
+    t.U = R;
+    t.ud = L - 1;
+    t.L = (R - L + 1 + LINE_VALUES - 1) / LINE_VALUES;
+    t.cl = c->wcl.as<CLine>();
+    t.pass = 2;
+    t.qlo = L;
+    t.qhi = R;
+    return t;
+}
+
+// Sieves window [L, R] into the window lines and advances the carry.
+mdr_status enqueue_window_table(mdr_ctx* c, const WindowPlan& w, i64 L, i64 R, const TableParam& res) {
+    const SieveShape& sh = SIEVE_SHAPES[w.shape];
+    const i64 seg = sh.seg;
+    const i64 s0 = (L - 1) / seg, s1 = (R + seg - 1) / seg;
+    const i64 NL = (R - L + 1 + LINE_VALUES - 1) / LINE_VALUES;
+    i64* carry = c->wcarry.as<i64>();
+    if (L == w.U0 + 1) {
+        hipLaunchKernelGGL(k_window_carry, dim3(1), dim3(64), 0, c->st, res, w.U0, carry, (const i64*)nullptr);
+        HIPT(hipGetLastError(), "launch window carry");
+    }
+    SieveArgs a;
+    a.U = R;
+    a.ud = L - 1;   // every value of the window lies above it: lines only
+    a.L = NL;
+    a.mu_segs = 0;  // mu comes from the resident table
+    a.stencil = c->stencil.as<uint8_t>();
+    a.wprimes = c->wprimes.as<uint32_t>();
+    a.wrcp = c->wrcp.as<double>();
+    a.wweights = c->wweights.as<uint8_t>();
+    a.n_w = c->n_w;
+    a.n_small = c->n_small;
+    a.sqprimes = c->sqprimes.as<uint32_t>();
+    a.n_sq = c->n_sq;
+    a.n_sq_small = c->n_sq_small;
+    a.Md = c->Md.as<int16_t>();
+    a.cl = c->wcl.as<CLine>();
+    a.mu6 = c->mu6.as<uint32_t>();
+    a.segsum = c->wseg.as<i64>();
+    a.seg0 = s0;
+    const i64 maxgrid = ((i64)1 << 31) / sh.threads;
+    for (i64 b0 = s0; b0 < s1; b0 += maxgrid) {
+        a.seg_first = b0;
+        const i64 nb = std::min(maxgrid, s1 - b0);
+        HIPT(launch_sieve(w.shape, (unsigned)nb, c->st, a), "launch window sieve");
+    }
+    HIPT(scan_exclusive_i64(c->wseg.as<i64>(), c->wseg.as<i64>(), s1 - s0, c->wscan.as<i64>(), scan_tmp_len(s1 - s0),
+                            carry + 1, c->st),
+         "scan window");
+    FixupArgs f;
+    f.ud = L - 1;
+    f.L = NL;
+    f.nd = 0;
+    f.seg_shift = sh.shift;
+    f.Md = c->Md.as<int16_t>();
+    f.cl = c->wcl.as<CLine>();
+    f.segoff = c->wseg.as<i64>();
+    f.seg0 = s0;
+    f.carry = carry;
+    const i64 fix_blocks = std::min<i64>((std::max<i64>(NL, 1) + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)fix_blocks), dim3(256), 0, c->st, f);
+    HIPT(hipGetLastError(), "launch window fixup");
+    hipLaunchKernelGGL(k_window_carry, dim3(1), dim3(64), 0, c->st, res, (i64)0, carry, (const i64*)(carry + 1));
+    HIPT(hipGetLastError(), "launch window carry");
     return MDR_OK;
 }
 
@@ -595,10 +727,41 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     // alone keep a dense table only up to 5e7 (where |M| fits the int16 tier)
     ud = any_sum ? std::min(U, ud) : U <= 50000000LL ? U : 1048512;
     if (ud < U) ud = (ud / LINE_VALUES) * LINE_VALUES;
+    // sieve workgroup shape: explicit, else the tuned one of the largest X
+    int shape = o.sieve_shape > 0 ? o.sieve_shape - 1 : 0;
+    if (need_table && o.sieve_shape == 0 && o.params == MDR_PARAMS_TUNED) {
+        i64 xmax = 1;
+        for (const XPlan& p : plans)
+            if (p.kind != 0) xmax = std::max(xmax, p.X);
+        shape = tune_sieve_shape(xmax);
+    }
+    // windowed table: the resident part holds whole segments up to what mu,
+    // the dense tier, the constants (A, B <= cx sqrt(X) < mu_limit) and the
+    // lookups read; the compressed tier above it streams through windows
+    WindowPlan win;
+    i64 wreq = o.window;   // explicit, else a tuned plan's
+    for (const XPlan& p : plans)
+        if (o.window == 0 && p.kind == 2) wreq = std::max(wreq, p.window);
+    if (need_table && any_sum && wreq > 0) {
+        const i64 seg = SIEVE_SHAPES[shape].seg;
+        i64 need = std::max(mu_lim, ud);
+        for (const XPlan& p : plans)
+            if (p.kind == 1) need = std::max(need, p.X);
+        const i64 U0 = std::max<i64>(seg, (need + seg - 1) / seg * seg);
+        if (U0 < U) {
+            win.U0 = U0;
+            win.U = U;
+            win.wvals = std::max<i64>(seg, (wreq + seg - 1) / seg * seg);
+            win.shape = shape;
+            U = U0;
+            if (ud > U) ud = U;
+        }
+    }
+    const bool windowed = win.count() > 0;
     // tuned calls: a cached table that covers the request serves it, and every
     // summed X takes the cached cutoff (M(X) does not depend on u)
-    if (need_table && !o.no_table_reuse && c->tab_valid && o.params == MDR_PARAMS_TUNED && o.tune_u == 0 &&
-        c->tU >= U && c->tmu >= mu_lim) {
+    if (need_table && !windowed && !o.no_table_reuse && c->tab_valid && o.params == MDR_PARAMS_TUNED &&
+        o.tune_u == 0 && c->tU >= U && c->tmu >= mu_lim) {
         for (XPlan& p : plans) {
             if (p.kind != 2 || p.u >= c->tU) continue;
             if (p.X <= c->tU) {
@@ -619,17 +782,14 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     BatchLaunch b;
     b.nx = nx;
     if (need_table) {
-        // sieve workgroup shape: explicit, else the tuned one of the largest X
-        int shape = o.sieve_shape > 0 ? o.sieve_shape - 1 : 0;
-        if (o.sieve_shape == 0 && o.params == MDR_PARAMS_TUNED) {
-            i64 xmax = 1;
-            for (const XPlan& p : plans)
-                if (p.kind != 0) xmax = std::max(xmax, p.X);
-            shape = tune_sieve_shape(xmax);
-        }
+        if (!windowed) c->wcl.release();   // a one-table call needs the memory
         mdr_status s = prepare_table(c, U, ud, mu_lim, !o.no_table_reuse, shape, b.tj);
         if (s != MDR_OK) return s;
         if (stats && !b.tj.build) stats->table_reused = 1;
+        if (windowed) {
+            s = prepare_windows(c, win);
+            if (s != MDR_OK) return s;
+        }
     }
     b.tp = c->tp();
 
@@ -731,7 +891,14 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
 
     // ---- launch ----
     bool instantiated = false;
-    if (o.use_graph) {
+    const bool graph = o.use_graph && !windowed;   // windowed calls run eagerly
+    double tab_ms = 0, part_ms = 0, quot_ms = 0, graph_ms = 0;
+    auto phase_ms = [&](int e0, int e1) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev[e0], c->ev[e1]);
+        return (double)ms;
+    };
+    if (graph) {
         mdr_status s = launch_graph(c, b, instantiated);
         if (s != MDR_OK) return s;
     } else {
@@ -740,6 +907,43 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     }
     HIPT(hipStreamSynchronize(c->st), "batch");
     if (b.tj.build) c->tab_valid = true;
+    if (graph) {
+        graph_ms = phase_ms(5, 6);
+    } else {
+        if (b.tj.build) tab_ms += phase_ms(0, 1);
+        if (b.nxs) {
+            part_ms += phase_ms(2, 3);
+            quot_ms += phase_ms(3, 4);
+        }
+    }
+    // windows of the compressed tier above the resident part: sieve the
+    // window, then a quotient pass over the lookups that fall in it
+    const TableParam rtp = b.tp;
+    for (i64 L = win.U0 + 1; windowed && L <= win.U; L += win.wvals) {
+        const i64 R = std::min(win.U, L + win.wvals - 1);
+        HIPT(hipEventRecord(c->ev[0], c->st), "event");
+        mdr_status s = enqueue_window_table(c, win, L, R, rtp);
+        if (s != MDR_OK) return s;
+        HIPT(hipEventRecord(c->ev[1], c->st), "event");
+        PartArgs pa = b.pa;
+        QuotArgs qa = b.qa;
+        pa.tp = window_param(c, rtp, L, R);
+        qa.tp = pa.tp;
+        hipLaunchKernelGGL(k_clear3, dim3(1), dim3(64), 0, c->st, (u64*)c->ctr.as<unsigned long long>(), (i64)1,
+                           (u64*)nullptr, (i64)0, (u64*)nullptr, (i64)0);   // the chunk queue
+        HIPT(hipGetLastError(), "launch clear");
+        s = enqueue_quotient(c, b, pa, qa, true);
+        if (s != MDR_OK) return s;
+        HIPT(hipEventRecord(c->ev[4], c->st), "event");
+        HIPT(hipStreamSynchronize(c->st), "window");
+        tab_ms += phase_ms(0, 1);
+        part_ms += phase_ms(2, 3);
+        quot_ms += phase_ms(3, 4);
+        if (stats) {
+            stats->windows += 1;
+            stats->table_bytes += (double)sizeof(CLine) * (double)pa.tp.L;
+        }
+    }
 
     // ---- results ----
     const auto td0 = std::chrono::steady_clock::now();
@@ -769,24 +973,15 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     }
     if (stats) {
         stats->d2h_copy_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - td0).count();
-        float ms = 0;
-        if (o.use_graph) {
-            (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
-            stats->graph_ms += ms;
+        if (graph) {
+            stats->graph_ms += graph_ms;
             stats->graph_used = 1;
             stats->graph_instantiated += instantiated ? 1 : 0;
-            stats->kernel_time += ms / 1000.0;
+            stats->kernel_time += graph_ms / 1000.0;
         } else {
-            if (b.tj.build) {
-                (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-                stats->table_ms += ms;
-            }
-            if (b.nxs) {
-                (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
-                stats->partition_ms += ms;
-                (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]);
-                stats->quotient_ms += ms;
-            }
+            stats->table_ms += tab_ms;
+            stats->partition_ms += part_ms;
+            stats->quotient_ms += quot_ms;
             stats->kernel_time = (stats->table_ms + stats->partition_ms + stats->quotient_ms) / 1000.0;
         }
         if (b.tj.build) stats->table_bytes += table_bytes(b.tj);
@@ -800,7 +995,7 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
                                     : o.partition == MDR_PARTITION_TERMS ? MDR_PARTITION_TERMS
                                                                          : MDR_PARTITION_STATIC;
         }
-        stats->U = c->tU;
+        stats->U = windowed ? win.U : c->tU;
         stats->u_dense = c->tud;
         stats->mu_limit = c->tmu;
         stats->W = c->tL;

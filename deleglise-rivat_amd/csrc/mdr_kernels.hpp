@@ -21,15 +21,19 @@ struct SieveArgs {
     int16_t* Md;
     CLine* cl;
     uint32_t* mu6;
-    i64* segsum;
+    i64* segsum;               // segsum[s - seg0]: segment s's sum
+    i64 seg0;
 };
 
 struct FixupArgs {
-    i64 ud, L;
+    i64 ud, L;                 // lines hold the values above ud
+    i64 nd;                    // dense values 1..nd to fix (0 for a window of lines)
     int seg_shift;             // segment length 3 * 2^seg_shift
     int16_t* Md;
     CLine* cl;
-    const i64* segoff;
+    const i64* segoff;         // segoff[s - seg0]: exclusive offset of segment s
+    i64 seg0;
+    const i64* carry;          // nullptr, or M(ud) added to every checkpoint
 };
 
 // Fixed per-slot cost (cycles) charged by the learned partition for a G

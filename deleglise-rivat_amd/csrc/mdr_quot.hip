@@ -421,7 +421,7 @@ __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
             }
             const Frame f = uniform_frame(make_frame(xp, a.tp.mu6, G - xp.slot_base));
             const i64 lo = (b0 > og ? b0 : og) - og, hi = (b1 < og1 ? b1 : og1) - og;
-            if (lo == 0 && lane == 0) {
+            if (lo == 0 && lane == 0 && a.tp.pass != 2) {   // a window pass has no constants
                 const i128 cst = (i128)f.coef * frame_const(f, a.tp);
                 add128(acc_lo, acc_hi, (u64)cst, (i64)(cst >> 64));
             }

@@ -247,7 +247,7 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
         const int32_t excl = inc - t;
 #pragma unroll
         for (int i = 0; i < GROUPS / 64; ++i) gpre[lane * (GROUPS / 64) + i] = excl + loc[i];
-        if (lane == 63) a.segsum[s] = inc;
+        if (lane == 63) a.segsum[s - a.seg0] = inc;
     }
     __syncthreads();
 
@@ -309,14 +309,15 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
 __global__ void k_table_fixup(FixupArgs a) {
     const i64 i0 = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     const i64 stride = (i64)gridDim.x * blockDim.x;
-    if (i0 == 0) a.Md[0] = 0;
-    for (i64 i = i0; i < a.ud; i += stride) {
+    if (i0 == 0 && a.nd > 0) a.Md[0] = 0;
+    for (i64 i = i0; i < a.nd; i += stride) {
         const i64 n = i + 1;
         const i64 s = ((n - 1) >> a.seg_shift) / 3;
-        a.Md[n] = (int16_t)(a.Md[n] + (int32_t)a.segoff[s]);
+        a.Md[n] = (int16_t)(a.Md[n] + (int32_t)a.segoff[s - a.seg0]);
     }
+    const int32_t carry = a.carry ? (int32_t)a.carry[0] : 0;
     for (i64 i = i0; i < a.L; i += stride) {   // line i: its three checkpoints
-        const int32_t add = (int32_t)a.segoff[((a.ud + (i64)LINE_VALUES * i) >> a.seg_shift) / 3];
+        const int32_t add = carry + (int32_t)a.segoff[((a.ud + (i64)LINE_VALUES * i) >> a.seg_shift) / 3 - a.seg0];
         int32_t* ck = a.cl[i].ck;
         ck[0] += add;
         ck[1] += add;

@@ -40,6 +40,7 @@ double g_batch_factor = 1.0;
 // Largest tuned cutoff: 4e11 values (133 GB of compressed lines), lowered to
 // what the device's memory holds (tune_set_device_memory).
 constexpr long long U_CAP_MAX = 400000000000LL;
+constexpr long long WINDOW_U_MAX = 1000000000000LL;
 std::atomic<long long> g_u_cap{U_CAP_MAX};
 
 void load() {
@@ -111,9 +112,17 @@ TuneChoice tune_lookup(i64 X, bool streaming) {
     u = std::max(u, 3 * S);
     const i64 ends = (i64)(cx * std::sqrt((double)X)) + 1000;
     u = std::max(u, ends + 64);
-    u = std::min<i64>(u, g_u_cap.load());
-    u = ((u + 191) / 192) * 192;
     TuneChoice t;
+    // above the one-table cap the tuned cutoff is kept (up to 1e12, where the
+    // quotient kernels' int32 partial sums stay exact) and the table is
+    // windowed: measured at X = 2^63 - 1 on MI355X, u = 1e12 in windows of
+    // 4e11 takes 19 % less GPU time than one 4e11 table
+    const i64 cap = g_u_cap.load();
+    if (u > cap) {
+        t.window = cap;
+        u = std::min<i64>(u, WINDOW_U_MAX);
+    }
+    u = ((u + 191) / 192) * 192;
     t.u = u;
     t.cx = cx;
     t.ud = std::min(u, (ud / 192) * 192);

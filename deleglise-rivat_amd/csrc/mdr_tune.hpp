@@ -13,6 +13,8 @@ struct TuneChoice {
     i64 u;      // sieve cutoff
     i64 ud;     // dense tier limit (multiple of 64)
     double cx;  // S1/S2 split factor
+    i64 window = 0;   // > 0: the cutoff exceeds what one table may hold; stream the
+                      // compressed tier through windows of this many values
 };
 
 TuneChoice tune_lookup(i64 X, bool streaming);

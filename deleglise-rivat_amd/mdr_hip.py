@@ -39,7 +39,8 @@ class Opts(ctypes.Structure):
                 ("sched", ctypes.c_int), ("host_threads", ctypes.c_int), ("chunk", ctypes.c_longlong),
                 ("record_cycles", ctypes.c_int), ("no_table_reuse", ctypes.c_int), ("tune_u", ctypes.c_longlong),
                 ("tune_ud", ctypes.c_longlong), ("tune_cx", ctypes.c_double),
-                ("use_graph", ctypes.c_int), ("tune_batch", ctypes.c_double), ("sieve_shape", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("tune_batch", ctypes.c_double), ("sieve_shape", ctypes.c_int),
+                ("window", ctypes.c_longlong)]
 
 
 class Stats(ctypes.Structure):
@@ -53,7 +54,8 @@ class Stats(ctypes.Structure):
                 ("cx", ctypes.c_double), ("s1_terms", ctypes.c_longlong), ("s2_terms", ctypes.c_longlong),
                 ("table_bytes", ctypes.c_double), ("s1_comp_terms", ctypes.c_longlong),
                 ("partition_used", ctypes.c_int), ("graph_used", ctypes.c_int),
-                ("graph_instantiated", ctypes.c_int), ("graph_ms", ctypes.c_double)]
+                ("graph_instantiated", ctypes.c_int), ("graph_ms", ctypes.c_double),
+                ("windows", ctypes.c_longlong)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -132,14 +134,14 @@ def _i128(lo, hi):
 def make_opts(strategy=STRATEGY_AUTO, cx=0.0, u_cap=0, fx=0.0, u_cap_dense=0, fx_set=0,
               u_cap_dense_set=0, partition=PARTITION_STATIC, params=PARAMS_REFERENCE,
               sched=SCHED_DYNAMIC, chunk=0, record_cycles=0, no_table_reuse=0, tune_u=0, tune_ud=0,
-              tune_cx=0.0, use_graph=0, tune_batch=0.0, sieve_shape=0):
+              tune_cx=0.0, use_graph=0, tune_batch=0.0, sieve_shape=0, window=0):
     o = Opts()
     load().mdr_default_opts(ctypes.byref(o))
     o.strategy, o.cx, o.u_cap, o.fx, o.u_cap_dense = strategy, cx, u_cap, fx, u_cap_dense
     o.fx_set, o.u_cap_dense_set, o.partition, o.params = fx_set, u_cap_dense_set, partition, params
     o.sched, o.chunk, o.record_cycles, o.no_table_reuse = sched, chunk, record_cycles, no_table_reuse
     o.tune_u, o.tune_ud, o.tune_cx = tune_u, tune_ud, tune_cx
-    o.use_graph, o.tune_batch, o.sieve_shape = use_graph, tune_batch, sieve_shape
+    o.use_graph, o.tune_batch, o.sieve_shape, o.window = use_graph, tune_batch, sieve_shape, window
     return o
 
 

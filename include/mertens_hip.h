@@ -90,6 +90,15 @@ typedef struct {
     int sieve_shape;          /* sieve segment length / workgroup: 0 = automatic (tuned with
                                  MDR_PARAMS_TUNED), 1 = 49152/512, 2 = 24576/256,
                                  3 = 98304/1024, 4 = 49152/256; the table is the same */
+    long long window;         /* windowed table (compute_mertens_windowed,
+                                 mertens_dr_windowed.hpp:49-407): 0 = the whole table is
+                                 resident; n > 0 = keep only mu, the dense tier and the
+                                 values up to max(cx sqrt(X) + 1000, X/u + 1000) resident and
+                                 stream the compressed tier above them through windows of
+                                 n values (rounded up to whole sieve segments); the result
+                                 is the same, the table's memory is the resident part plus
+                                 one window, and the cutoff may exceed the one-table limit
+                                 (up to tune_u <= 1e12); such calls run eagerly (no graph) */
 } mdr_opts;
 
 typedef struct {
@@ -114,6 +123,7 @@ typedef struct {
     int graph_instantiated;              /* graphs instantiated (not updated in place) */
     double graph_ms;                     /* GPU time of the graph launches; the per-phase
                                             times stay 0 for graph calls */
+    long long windows;                   /* compressed-tier windows streamed (windowed calls) */
 } mdr_stats;
 
 /* One cycle-log record: a run of S1 (n) or S2 (j) positions of one G(y)

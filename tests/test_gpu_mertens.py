@@ -115,6 +115,15 @@ def test_dropin_cpp_program():
     assert r.returncode == 0, r.stdout + r.stderr
 
 
+def test_dropin_windowed_cpp_program():
+    exe = os.path.join(ROOT, "tests", "cpp", "test_dropin_windowed")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    r = subprocess.run([exe, str(10**16)], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "[win]" in r.stderr
+
+
 def test_largest_table_and_external_1e18():
     """M(1e18) = -46758740 is not a reference value (OEIS A084237): a
     cross-check at the largest X of the batch workloads, with the tuned cutoff
@@ -154,13 +163,16 @@ def test_cli_modes(known):
 
 def test_largest_int64_x_two_cutoffs_agree():
     """X = 2^63 - 1, the top of the int64 range (no reference value exists):
-    the tuned cutoff (4e11) and a 3e11 cutoff must give the same M(X); the
-    64-bit S2 path (j >= 2^31) and the largest quotients are exercised."""
+    the tuned plan (cutoff 1e12 above the one-table cap, so a windowed table)
+    and a 3e11 one-table cutoff must give the same M(X); the 64-bit S2 path
+    (j >= 2^31) and the largest quotients are exercised."""
     e = mdr_hip.Engine(0)
     try:
         X = 2**63 - 1
         a = e.mertens(X, params=mdr_hip.PARAMS_TUNED, no_table_reuse=1)
+        assert e.last_stats.windows >= 2 and abs(e.last_stats.u - 10**12) < 192
         b = e.mertens(X, params=mdr_hip.PARAMS_TUNED, tune_u=3 * 10**11, tune_ud=524160, no_table_reuse=1)
-        assert a == b
+        assert e.last_stats.windows == 0
+        assert a == b == 580197744
     finally:
         e.close()
