@@ -4,15 +4,18 @@
 //   mdr_bench <X> [threads]                      M(X) with the timing breakdown
 //   mdr_bench --benchmark [max_exp] [threads] [repeats]
 //                                                sweep 10^1..10^max_exp (median time)
-//   mdr_bench --compare [max_exp] [threads]      reference parameters vs the tuned
-//                                                GPU split, 10^10..10^max_exp
+//   mdr_bench --compare [max_exp] [threads]      Dirichlet DP engine vs Deleglise-Rivat,
+//                                                10^10..10^max_exp (bench_dr.cpp:76-122)
+//   mdr_bench --compare-split [max_exp]          reference parameter rules vs the tuned
+//                                                GPU split
 //
 // It calls the drop-in header, so the reference's environment controls
 // (MERTENS_USE_CHMT, MERTENS_CX, MERTENS_CHMT_U_CAP, MERTENS_FX,
-// MERTENS_U_CAP) and MERTENS_HIP_TUNED / MERTENS_HIP_LEARNED apply.  The
-// reference's --compare ran its CPU engine beside the GPU; this engine ships
-// no CPU path, so --compare sets the reference's parameter rules against the
-// autotuned split, both exact on the GPU.  threads is accepted and ignored.
+// MERTENS_U_CAP) and MERTENS_HIP_TUNED / MERTENS_HIP_LEARNED apply.
+// --compare times the Dirichlet cross-check engine (dirichlet_engine.hpp, on
+// the GPU here) against the Deleglise-Rivat engine and stops if they differ,
+// as the reference's assert does.  threads is accepted and ignored.
+#include "dirichlet_engine.hpp"
 #include "mertens_dr.hpp"
 
 #include <algorithm>
@@ -29,7 +32,8 @@ void usage(const char* prog) {
     std::cout << "Usage:\n"
               << "  " << prog << " <X> [threads]                                M(X) at a single value\n"
               << "  " << prog << " --benchmark [max_exp] [threads] [repeats]   sweep (median time)\n"
-              << "  " << prog << " --compare [max_exp] [threads]               reference vs tuned GPU split\n";
+              << "  " << prog << " --compare [max_exp] [threads]               Dirichlet DP vs Deleglise-Rivat\n"
+              << "  " << prog << " --compare-split [max_exp]                   reference vs tuned GPU split\n";
 }
 
 using mertens_dr::int64;
@@ -103,6 +107,37 @@ int main(int argc, char** argv) {
             return 0;
         }
         if (a1 == "--compare") {
+            const int max_exp = argc >= 3 ? std::stoi(argv[2]) : 14;
+            if (max_exp < 10 || max_exp > 18) {
+                std::cerr << "max_exp must be 10..18." << std::endl;
+                return 1;
+            }
+            std::cout << "==========================================================================================================\n"
+                      << "  Comparative Performance Benchmark: O(X^(3/4)) Dirichlet DP vs. O(X^(2/3)) Deleglise-Rivat Engine\n"
+                      << "==========================================================================================================\n"
+                      << "| Target X    | Exact M(X)        | Dirichlet DP Time | Deleglise-Rivat Time | Speedup (vs DP) |\n"
+                      << "| :---        | :---              | :---              | :---                 | :---            |\n";
+            int64 X = 10000000000LL;
+            for (int e = 10; e <= max_exp; ++e) {
+                mdr_stats st{};
+                double dr_time = 0;
+                const int64 dr_val = eval(X, &st, &dr_time);
+                const auto t2 = std::chrono::steady_clock::now();
+                const int64 dp_val = dirichlet::DirichletEngine::compute_mertens(X, 0);
+                const double dp_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t2).count();
+                if (dr_val != dp_val) {
+                    std::cerr << "M(" << X << "): Deleglise-Rivat " << dr_val << " vs Dirichlet " << dp_val << std::endl;
+                    return 2;
+                }
+                std::cout << "| 10^" << std::setw(2) << std::left << e << "      | " << std::setw(17) << std::right
+                          << dr_val << " | " << std::setw(15) << std::fixed << std::setprecision(4) << dp_time
+                          << " s | " << std::setw(18) << dr_time << " s | " << std::setw(14) << std::setprecision(2)
+                          << dp_time / dr_time << "x |\n";
+                if (e < 18) X *= 10;
+            }
+            return 0;
+        }
+        if (a1 == "--compare-split") {
             const int max_exp = argc >= 3 ? std::stoi(argv[2]) : 16;
             if (max_exp < 10 || max_exp > 18) {
                 std::cerr << "max_exp must be 10..18." << std::endl;

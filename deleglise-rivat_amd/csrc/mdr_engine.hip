@@ -95,6 +95,8 @@ struct mdr_ctx {
     // windows of the compressed tier (windowed tables): lines, segment sums,
     // scan scratch, [carry M(window start - 1), window total]
     DevBuf wcl, wseg, wscan, wcarry;
+    // Dirichlet cross-check engine: M(x/k) for the large quotients
+    DevBuf big;
     // work
     DevBuf xparam, slot_len, slot_cost, bounds, out128, ctr, stamps, comp_out, lookup;
     std::vector<StampRec> last_stamps;
@@ -1011,6 +1013,89 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
 
 }  // namespace
 
+// ---- Dirichlet cross-check engine -----------------------------------------------------
+// M(x) from sum_{d <= v} M(floor(v/d)) = 1, an algorithm independent of the
+// Deleglise-Rivat split (the reference's optional dirichlet_engine.hpp,
+// bench_dr.cpp:9-12, test_dr.cpp:8-11 and :71-88, is not shipped with it).  With
+// t = isqrt(v) and the d grouped by q = floor(v/d) <= t,
+//   M(v) = 1 - sum_{2 <= d <= v/(t+1)} M(floor(v/d)) - sum_{q <= t} (floor(v/q) - floor(v/(q+1))) M(q).
+// M(q) for q <= L comes from a table (L about x^(2/3)); the large values
+// v = floor(x/k), k <= K = x/(L+1), are computed from large k down: M(x/k)
+// needs M(x/(k d)) only for k d <= K, so every k in (K/2^(j+1), K/2^j] can run
+// in parallel once the levels above it are done (one workgroup per k).
+
+__global__ __launch_bounds__(256) void k_dirichlet_level(TableParam tp, i64 x, i64 klo, i64 khi, i64* big) {
+    __shared__ i64 red[4];
+    const i64 k = khi - (i64)blockIdx.x;
+    if (k <= klo) return;
+    const i64 v = div_exact(x, k);
+    i64 t = (i64)__builtin_sqrt((double)v);
+    while ((u64)t * (u64)t > (u64)v) --t;
+    while ((u64)(t + 1) * (u64)(t + 1) <= (u64)v) ++t;
+    const i64 n1 = div_exact(v, t + 1);
+    const double dv = (double)v;
+    i64 s = 0;
+    for (i64 d = 2 + (i64)threadIdx.x; d <= n1; d += blockDim.x) {
+        const i64 q = (i64)qdiv_big((u64)v, dv, (u64)d, rcp_fast((double)d));
+        s += q <= tp.U ? (i64)get_M(tp, q) : big[k * d];
+    }
+    for (i64 q = 1 + (i64)threadIdx.x; q <= t; q += blockDim.x) {
+        const i64 a = (i64)qdiv_big((u64)v, dv, (u64)q, rcp_fast((double)q));
+        const i64 b = (i64)qdiv_big((u64)v, dv, (u64)(q + 1), rcp_fast((double)(q + 1)));
+        s += (a - b) * (i64)get_M(tp, q);
+    }
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) big[k] = 1 - (red[0] + red[1] + red[2] + red[3]);
+}
+
+__global__ void k_dirichlet_small(TableParam tp, i64 x, i64* big) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) big[1] = get_M(tp, x);
+}
+
+mdr_status dirichlet_mertens(mdr_ctx* c, i64 X, long long* out, mdr_stats* stats) {
+    if (X < 1) { *out = 0; return MDR_OK; }
+    // table to L ~ x^(2/3) (at least sqrt(x) + 1, at most 2e10: the large
+    // values then take 4 x / sqrt(L) term evaluations)
+    const i64 r = isqrt_host(X);
+    i64 L = (i64)std::ceil(std::pow((double)X, 2.0 / 3.0));
+    L = std::max<i64>({L, r + 1, 100000});
+    L = std::min<i64>({L, 20000000000LL, X});
+    L = std::max<i64>(L, std::min<i64>(X, r + 1));
+    const i64 ud = L <= 1048512 ? L : 1048512;
+    mdr_status s = build_table(c, L, ud, 1, stats);
+    if (s != MDR_OK) return s;
+    const TableParam tp = c->tp();
+    const i64 K = X / (L + 1);
+    HIPT(c->big.ensure((size_t)(K + 2) * 8), "alloc large values");
+    i64* big = c->big.as<i64>();
+    HIPT(hipEventRecord(c->ev[3], c->st), "event");
+    if (K == 0) {
+        hipLaunchKernelGGL(k_dirichlet_small, dim3(1), dim3(64), 0, c->st, tp, X, big);
+        HIPT(hipGetLastError(), "launch dirichlet");
+    }
+    for (i64 khi = K; khi >= 1; khi /= 2) {
+        const i64 klo = khi / 2;
+        hipLaunchKernelGGL(k_dirichlet_level, dim3((unsigned)(khi - klo)), dim3(256), 0, c->st, tp, X, klo, khi, big);
+        HIPT(hipGetLastError(), "launch dirichlet level");
+    }
+    HIPT(hipEventRecord(c->ev[4], c->st), "event");
+    long long m = 0;
+    HIPT(d2h(c, &m, big + 1, 8), "download");
+    *out = m;
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]);
+        stats->quotient_ms += ms;
+        stats->kernel_time = (stats->table_ms + stats->quotient_ms) / 1000.0;
+        stats->U = L;
+        stats->u = L;
+        stats->N = K;
+    }
+    return MDR_OK;
+}
+
 // ---- C ABI --------------------------------------------------------------------------
 extern "C" {
 
@@ -1125,6 +1210,18 @@ mdr_status mdr_mertens_batch(mdr_ctx* c, const long long* X, size_t n, const mdr
 
 mdr_status mdr_mertens(mdr_ctx* c, long long X, const mdr_opts* opts, long long* out, mdr_stats* stats) {
     return mdr_mertens_batch(c, &X, 1, opts, out, stats);
+}
+
+mdr_status mdr_mertens_dirichlet(mdr_ctx* c, long long X, long long* out, mdr_stats* stats) {
+    if (!c || !out) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    if (stats) *stats = mdr_stats{};
+    const auto t0 = std::chrono::steady_clock::now();
+    mdr_status s = dirichlet_mertens(c, X, out, stats);
+    if (stats) stats->total_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return s;
 }
 
 mdr_status mdr_table_build(mdr_ctx* c, long long U, long long u_dense, long long mu_limit) {

@@ -22,7 +22,7 @@ STATUS = {0: "MDR_OK", 1: "MDR_ERR_INVALID", 2: "MDR_ERR_HIP", 3: "MDR_ERR_NOMEM
 
 # Every symbol include/mertens_hip.h declares.
 EXPORTS = ["mdr_default_opts", "mdr_init", "mdr_shutdown", "mdr_last_error", "mdr_mertens",
-           "mdr_mertens_batch", "mdr_plan", "mdr_table_build", "mdr_table_read_M",
+           "mdr_mertens_batch", "mdr_mertens_dirichlet", "mdr_plan", "mdr_table_build", "mdr_table_read_M",
            "mdr_table_read_mu", "mdr_s2_interval", "mdr_s1_sum", "mdr_g_eval",
            "mdr_cost_model_set", "mdr_cost_model_get", "mdr_cost_model_load", "mdr_cost_model_save",
            "mdr_cycle_log", "mdr_mlp_features", "mdr_mlp_predict", "mdr_mlp_grad", "mdr_mlp_train",
@@ -96,6 +96,8 @@ def load():
     lib.mdr_mertens.restype = i32
     lib.mdr_mertens_batch.argtypes = [vp, P(ll), ctypes.c_size_t, P(Opts), P(ll), P(Stats)]
     lib.mdr_mertens_batch.restype = i32
+    lib.mdr_mertens_dirichlet.argtypes = [vp, ll, P(ll), P(Stats)]
+    lib.mdr_mertens_dirichlet.restype = i32
     lib.mdr_plan.argtypes = [vp, ll, P(Opts), P(i32), P(ll), P(ll), P(ll), P(ll), P(dbl)]
     lib.mdr_plan.restype = i32
     lib.mdr_table_build.argtypes = [vp, ll, ll, ll]
@@ -225,6 +227,16 @@ class Engine:
         stats = Stats()
         self._check(self.lib.mdr_mertens(self.ctx, X, ctypes.byref(o), ctypes.byref(out), ctypes.byref(stats)),
                     f"mdr_mertens({X})")
+        self.last_stats = stats
+        return out.value
+
+    def dirichlet(self, X):
+        """M(X) from the Dirichlet cross-check engine (independent of the
+        Deleglise-Rivat split)."""
+        out = ctypes.c_longlong()
+        stats = Stats()
+        self._check(self.lib.mdr_mertens_dirichlet(self.ctx, X, ctypes.byref(out), ctypes.byref(stats)),
+                    f"mdr_mertens_dirichlet({X})")
         self.last_stats = stats
         return out.value
 

@@ -149,6 +149,15 @@ mdr_status mdr_mertens(mdr_ctx* ctx, long long X, const mdr_opts* opts, long lon
 /* M(X[i]) for n values sharing one table sized to the largest cutoff. */
 mdr_status mdr_mertens_batch(mdr_ctx* ctx, const long long* X, size_t n, const mdr_opts* opts,
                              long long* out, mdr_stats* stats);
+/* Cross-check engine: dirichlet::DirichletEngine::compute_mertens (the
+ * reference's optional dirichlet_engine.hpp, included by bench_dr.cpp:9-12 and
+ * test_dr.cpp:8-11 and cross-validated at test_dr.cpp:71-88; not shipped with
+ * it).  Exact M(X) from sum_{d <= v} M(floor(v/d)) = 1 over the quotients
+ * v = floor(X/k), with a table to about X^(2/3) (at most 2e10): an algorithm
+ * independent of the Deleglise-Rivat split, about 4 X / sqrt(table) term
+ * evaluations.  stats: table_ms, quotient_ms (the large values), U = the
+ * table limit, N = the large values computed. */
+mdr_status mdr_mertens_dirichlet(mdr_ctx* ctx, long long X, long long* out, mdr_stats* stats);
 
 /* Waves the persistent quotient kernel keeps resident on the context's GPU
  * (the natural chunk count of a static round-robin schedule). */

@@ -155,10 +155,12 @@ def test_cli_modes(known):
     for e, row in zip(range(1, 14), rows):
         cells = [c.strip() for c in row.strip("|").split("|")]
         assert int(cells[1]) == 10**e and int(cells[2]) == known[10**e], row
-    r = subprocess.run([exe, "--compare", "14"], capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr
-    rows = [l for l in r.stdout.splitlines() if l.startswith("| 10^")]
-    assert [int(l.strip("|").split("|")[1]) for l in rows] == [known[10**e] for e in range(10, 15)]
+    for mode, top in (("--compare", 15), ("--compare-split", 14)):
+        r = subprocess.run([exe, mode, str(top)], capture_output=True, text=True, timeout=300)
+        print(r.stdout)
+        assert r.returncode == 0, r.stderr
+        rows = [l for l in r.stdout.splitlines() if l.startswith("| 10^")]
+        assert [int(l.strip("|").split("|")[1]) for l in rows] == [known[10**e] for e in range(10, top + 1)]
 
 
 def test_largest_int64_x_two_cutoffs_agree():
