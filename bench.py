@@ -263,6 +263,9 @@ def main():
                      "quotient": round(st.quotient_ms, 2)},
         "quotient_positions": {"s1_dense": st.s1_terms - st.s1_comp_terms, "s1_comp": st.s1_comp_terms,
                                "s2": st.s2_terms},
+        "cost_model": {"predicted_cycles": st.predicted_cycles, "measured_wave_cycles": st.measured_cycles,
+                       "predicted_over_measured": (round(st.predicted_cycles / st.measured_cycles, 3)
+                                                   if st.measured_cycles > 0 else None)},
         "roofline": roofline(st_l),
         "roofline_table": {"bound": "hbm", "achieved": round(st.table_bytes / (t_table / 1000) / 1e9, 2),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",

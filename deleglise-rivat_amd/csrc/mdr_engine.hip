@@ -80,6 +80,7 @@ struct mdr_ctx {
     hipStream_t st = nullptr;
     int ncu = 0;
     int quot_blocks_per_cu = 1;
+    int clock_khz = 0;
     std::mutex mu;
     std::string err;
     hipEvent_t ev[8] = {};
@@ -894,7 +895,7 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     // ---- launch ----
     bool instantiated = false;
     const bool graph = o.use_graph && !windowed;   // windowed calls run eagerly
-    double tab_ms = 0, part_ms = 0, quot_ms = 0, graph_ms = 0;
+    double tab_ms = 0, part_ms = 0, quot_ms = 0, graph_ms = 0, pred_cycles = 0;
     auto phase_ms = [&](int e0, int e1) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev[e0], c->ev[e1]);
@@ -909,6 +910,11 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     }
     HIPT(hipStreamSynchronize(c->st), "batch");
     if (b.tj.build) c->tab_valid = true;
+    if (b.learned && windowed) {   // the resident pass's predicted total
+        long long pc = 0;
+        HIPT(d2h(c, &pc, c->ctr.as<unsigned long long>() + 5, 8), "download");
+        pred_cycles += (double)pc;
+    }
     if (graph) {
         graph_ms = phase_ms(5, 6);
     } else {
@@ -938,6 +944,11 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
         if (s != MDR_OK) return s;
         HIPT(hipEventRecord(c->ev[4], c->st), "event");
         HIPT(hipStreamSynchronize(c->st), "window");
+        if (b.learned) {   // this pass's predicted total (the scan overwrites it per pass)
+            long long pc = 0;
+            HIPT(d2h(c, &pc, c->ctr.as<unsigned long long>() + 5, 8), "download");
+            pred_cycles += (double)pc;
+        }
         tab_ms += phase_ms(0, 1);
         part_ms += phase_ms(2, 3);
         quot_ms += phase_ms(3, 4);
@@ -993,6 +1004,10 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
             stats->s1_terms += (long long)(ch[1] + ch[2]);
             stats->s1_comp_terms += (long long)ch[2];
             stats->s2_terms += (long long)ch[3];
+            stats->predicted_cycles += windowed ? pred_cycles : b.learned ? (double)(long long)ch[5] : 0.0;
+            stats->measured_cycles += (graph ? 0.0 : quot_ms) * (double)c->clock_khz *
+                                      (double)((i64)c->ncu * c->quot_blocks_per_cu * 4);
+            stats->quotient_bytes += 2.0 * (double)ch[1] + 20.0 * (double)ch[2] + 3.0 * (double)ch[3];
             stats->partition_used = b.learned ? MDR_PARTITION_LEARNED
                                     : o.partition == MDR_PARTITION_TERMS ? MDR_PARTITION_TERMS
                                                                          : MDR_PARTITION_STATIC;
@@ -1120,6 +1135,7 @@ mdr_status mdr_init(mdr_ctx** out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->clock_khz, hipDeviceAttributeClockRate, device);
     int occ = 0;
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_quot, 256, 0);
     if (e != hipSuccess) {

@@ -55,7 +55,8 @@ class Stats(ctypes.Structure):
                 ("table_bytes", ctypes.c_double), ("s1_comp_terms", ctypes.c_longlong),
                 ("partition_used", ctypes.c_int), ("graph_used", ctypes.c_int),
                 ("graph_instantiated", ctypes.c_int), ("graph_ms", ctypes.c_double),
-                ("windows", ctypes.c_longlong)]
+                ("windows", ctypes.c_longlong), ("predicted_cycles", ctypes.c_double),
+                ("measured_cycles", ctypes.c_double), ("quotient_bytes", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

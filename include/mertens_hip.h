@@ -124,6 +124,11 @@ typedef struct {
     double graph_ms;                     /* GPU time of the graph launches; the per-phase
                                             times stay 0 for graph calls */
     long long windows;                   /* compressed-tier windows streamed (windowed calls) */
+    /* quotient phase against its models: the cost model's predicted cycles (learned
+     * partition; summed over the G evaluations), the measured wave-cycles
+     * (quotient_ms x shader clock x resident waves) and the algorithmic bytes
+     * (2 B per dense S1 lookup, 20 B per compressed one, 3 B per S2 position) */
+    double predicted_cycles, measured_cycles, quotient_bytes;
 } mdr_stats;
 
 /* One cycle-log record: a run of S1 (n) or S2 (j) positions of one G(y)

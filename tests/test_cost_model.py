@@ -121,6 +121,24 @@ def test_invalid_model_rejected_without_device():
 
 
 @pytest.mark.gpu
+def test_stats_report_cycles_and_algorithmic_bytes(engine, known):
+    # the learned partition's predicted total against the measured
+    # wave-cycles of the quotient phase, and the algorithmic bytes
+    X = 10**16
+    assert engine.mertens(X, partition=mdr_hip.PARTITION_LEARNED, params=mdr_hip.PARAMS_TUNED,
+                          no_table_reuse=1) == known[X]
+    st = engine.last_stats
+    dense = st.s1_terms - st.s1_comp_terms
+    assert st.quotient_bytes == 2 * dense + 20 * st.s1_comp_terms + 3 * st.s2_terms
+    assert st.measured_cycles > 0
+    if st.partition_used == mdr_hip.PARTITION_LEARNED:
+        print("predicted / measured", st.predicted_cycles / st.measured_cycles)
+        assert 0.25 < st.predicted_cycles / st.measured_cycles < 4.0
+    assert engine.mertens(X, params=mdr_hip.PARAMS_TUNED, no_table_reuse=1) == known[X]
+    assert engine.last_stats.predicted_cycles == 0
+
+
+@pytest.mark.gpu
 def test_learned_partition_beats_the_term_partition(engine, known):
     # What the trained network gains over the reference GPU engine's static
     # unit of work (whole G evaluations per chunk): at 1e16 the heaviest term
