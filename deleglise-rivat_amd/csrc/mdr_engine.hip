@@ -1071,12 +1071,14 @@ __global__ void k_dirichlet_small(TableParam tp, i64 x, i64* big) {
 
 mdr_status dirichlet_mertens(mdr_ctx* c, i64 X, long long* out, mdr_stats* stats) {
     if (X < 1) { *out = 0; return MDR_OK; }
-    // table to L ~ x^(2/3) (at least sqrt(x) + 1, at most 2e10: the large
-    // values then take 4 x / sqrt(L) term evaluations)
+    // the large values take about 4 x / sqrt(L) term evaluations (about 1.5e11
+    // per second): a cross-check engine, limited to X <= 1e18
+    if (X > 1000000000000000000LL) return fail(c, MDR_ERR_RANGE, "the Dirichlet engine takes X <= 1e18");
+    // table to L ~ x^(2/3) (at least sqrt(x) + 1, at most the one-table cap)
     const i64 r = isqrt_host(X);
     i64 L = (i64)std::ceil(std::pow((double)X, 2.0 / 3.0));
     L = std::max<i64>({L, r + 1, 100000});
-    L = std::min<i64>({L, 20000000000LL, X});
+    L = std::min<i64>({L, tune_u_cap(), X});
     L = std::max<i64>(L, std::min<i64>(X, r + 1));
     const i64 ud = L <= 1048512 ? L : 1048512;
     mdr_status s = build_table(c, L, ud, 1, stats);

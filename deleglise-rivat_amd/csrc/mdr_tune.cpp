@@ -163,6 +163,8 @@ int tune_sieve_shape(i64 X) {
     return 0;
 }
 
+i64 tune_u_cap() { return g_u_cap.load(); }
+
 void tune_set_device_memory(size_t bytes) {
     // compressed lines take 1/3 byte per value: 45 % of the memory at most
     const double cap = std::min((double)U_CAP_MAX, (double)bytes * 0.45 * 3.0);

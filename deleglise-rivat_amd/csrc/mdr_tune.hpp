@@ -25,6 +25,7 @@ i64 tune_batch_u(double xsum, double factor = 0);
 int tune_sieve_shape(i64 X);   // tuned sieve workgroup shape for X (index into SIEVE_SHAPES)
 // Lowers the largest tuned cutoff to what a device of this memory holds.
 void tune_set_device_memory(size_t bytes);
+i64 tune_u_cap();   // largest one-table cutoff for this device
 
 }  // namespace mdr
 

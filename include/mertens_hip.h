@@ -158,10 +158,11 @@ mdr_status mdr_mertens_batch(mdr_ctx* ctx, const long long* X, size_t n, const m
  * reference's optional dirichlet_engine.hpp, included by bench_dr.cpp:9-12 and
  * test_dr.cpp:8-11 and cross-validated at test_dr.cpp:71-88; not shipped with
  * it).  Exact M(X) from sum_{d <= v} M(floor(v/d)) = 1 over the quotients
- * v = floor(X/k), with a table to about X^(2/3) (at most 2e10): an algorithm
- * independent of the Deleglise-Rivat split, about 4 X / sqrt(table) term
- * evaluations.  stats: table_ms, quotient_ms (the large values), U = the
- * table limit, N = the large values computed. */
+ * v = floor(X/k), with a table to about X^(2/3) (at most the one-table cap):
+ * an algorithm independent of the Deleglise-Rivat split, about
+ * 4 X / sqrt(table) term evaluations.  X > 1e18 fails with MDR_ERR_RANGE.
+ * stats: table_ms, quotient_ms (the large values), U = the table limit,
+ * N = the large values computed. */
 mdr_status mdr_mertens_dirichlet(mdr_ctx* ctx, long long X, long long* out, mdr_stats* stats);
 
 /* Waves the persistent quotient kernel keeps resident on the context's GPU

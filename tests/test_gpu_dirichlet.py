@@ -52,6 +52,12 @@ def test_dirichlet_matches_dr_at_table_edges(engine):
         assert engine.dirichlet(X) == engine.mertens(X, params=mdr_hip.PARAMS_TUNED), X
 
 
+def test_dirichlet_at_1e17_and_its_range_limit(engine, known):
+    assert engine.dirichlet(10**17) == known[10**17]
+    with pytest.raises(mdr_hip.MdrError, match="RANGE"):
+        engine.dirichlet(10**18 + 1)
+
+
 def test_dirichlet_cpp_cross_validation_program():
     exe = os.path.join(ROOT, "tests", "cpp", "test_dropin_dirichlet")
     assert os.path.exists(exe), "built by __graft_entry__.build()"

@@ -103,6 +103,18 @@ def test_windows_sieve_with_primes_up_to_the_full_cutoff():
         assert e.last_stats.windows == 4
 
 
+def test_windowed_call_runs_eagerly_and_logs_cycles(engine, known):
+    # a graph request is run eagerly when the table is windowed; the cycle log
+    # collects every pass
+    X = 10**15
+    o = opts(params=mdr_hip.PARAMS_TUNED, window=5 * 10**8, use_graph=1, record_cycles=8)
+    assert engine.mertens(X, o) == known[X]
+    st = engine.last_stats
+    assert st.windows >= 2 and st.graph_used == 0
+    kinds = {s.kind for s in engine.cycle_log()}
+    assert {2, 3} & kinds and any(k >= 4 for k in kinds)   # compressed S1 and S2 runs
+
+
 def test_negative_window_is_rejected(engine):
     with pytest.raises(mdr_hip.MdrError):
         engine.mertens(10**12, opts(window=-1))
