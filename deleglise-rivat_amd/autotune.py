@@ -100,9 +100,10 @@ def tune_decade(T, d, waves, quick):
     # second pass on fx around the choice (the best cutoff moves with cx and the tier)
     f = cfg["fx"]
     T.sweep(xs, cfg, "fx", sorted({round(f * m, 3) for m in (0.8, 1.0, 1.25)}))
-    # sieve segment length / workgroup (0 = 49152/512, 1 = 24576/256,
-    # 2 = 98304/1024, 3 = 49152/256): the table is the same, only its build time moves
-    T.sweep(xs, cfg, "sieve", [0, 1, 2, 3] if not quick else [0, 1])
+    # sieve segment length / workgroup (0 = 49152/1024, 1 = 24576/256,
+    # 2 = 98304/1024, 3 = 49152/256, 4 = 49152/512, 5 = 24576/512): the table is
+    # the same, only its build time moves
+    T.sweep(xs, cfg, "sieve", [0, 1, 2, 3, 4, 5] if not quick else [0, 5])
     if d >= 14:
         t = T.sweep(xs, cfg, "chunks", chs)
     else:   # small calls: the automatic count (it adapts to the work) wins

@@ -84,8 +84,9 @@ struct QuotArgs {
 struct SieveShape {
     int seg, threads, shift;
 };
-constexpr SieveShape SIEVE_SHAPES[] = {{49152, 512, 14}, {24576, 256, 13}, {98304, 1024, 15}, {49152, 256, 14}};
-constexpr int N_SIEVE_SHAPES = 4;
+constexpr SieveShape SIEVE_SHAPES[] = {{49152, 1024, 14}, {24576, 256, 13}, {98304, 1024, 15},
+                                       {49152, 256, 14},  {49152, 512, 14},  {24576, 512, 13}};
+constexpr int N_SIEVE_SHAPES = 6;
 constexpr int SEG_MAX = 98304;
 hipError_t launch_sieve(int shape, unsigned blocks, hipStream_t st, const SieveArgs& a);
 __global__ void k_table_fixup(FixupArgs a);

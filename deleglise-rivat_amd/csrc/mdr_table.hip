@@ -327,10 +327,12 @@ __global__ void k_table_fixup(FixupArgs a) {
 
 hipError_t launch_sieve(int shape, unsigned blocks, hipStream_t st, const SieveArgs& a) {
     switch (shape) {
-        case 0: hipLaunchKernelGGL((k_sieve_t<49152, 512>), dim3(blocks), dim3(512), 0, st, a); break;
+        case 0: hipLaunchKernelGGL((k_sieve_t<49152, 1024>), dim3(blocks), dim3(1024), 0, st, a); break;
         case 1: hipLaunchKernelGGL((k_sieve_t<24576, 256>), dim3(blocks), dim3(256), 0, st, a); break;
         case 2: hipLaunchKernelGGL((k_sieve_t<98304, 1024>), dim3(blocks), dim3(1024), 0, st, a); break;
         case 3: hipLaunchKernelGGL((k_sieve_t<49152, 256>), dim3(blocks), dim3(256), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_sieve_t<49152, 512>), dim3(blocks), dim3(512), 0, st, a); break;
+        case 5: hipLaunchKernelGGL((k_sieve_t<24576, 512>), dim3(blocks), dim3(512), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

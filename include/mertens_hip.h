@@ -88,8 +88,9 @@ typedef struct {
     double tune_batch;        /* batch cutoff factor for MDR_PARAMS_TUNED batches
                                  (0 = from the tuning cache; accepted 0.05..20) */
     int sieve_shape;          /* sieve segment length / workgroup: 0 = automatic (tuned with
-                                 MDR_PARAMS_TUNED), 1 = 49152/512, 2 = 24576/256,
-                                 3 = 98304/1024, 4 = 49152/256; the table is the same */
+                                 MDR_PARAMS_TUNED), 1 = 49152/1024, 2 = 24576/256,
+                                 3 = 98304/1024, 4 = 49152/256, 5 = 49152/512,
+                                 6 = 24576/512; the table is the same */
     long long window;         /* windowed table (compute_mertens_windowed,
                                  mertens_dr_windowed.hpp:49-407): 0 = the whole table is
                                  resident; n > 0 = keep only mu, the dense tier and the

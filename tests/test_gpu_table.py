@@ -95,7 +95,7 @@ def test_compressed_tier_limit_is_an_error(engine):
                                                  tune_ud=ud, no_table_reuse=1))
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6])
 def test_sieve_shapes_build_the_same_table(engine, mu_ref, pre_ref, known, shape):
     # every sieve workgroup shape (segment length / threads) builds the same
     # table: a call leaves its table in the context, whose M and mu windows
@@ -115,4 +115,4 @@ def test_sieve_shapes_build_the_same_table(engine, mu_ref, pre_ref, known, shape
 
 def test_sieve_shape_out_of_range(engine):
     with pytest.raises(mdr_hip.MdrError):
-        engine.mertens(10**12, sieve_shape=5)
+        engine.mertens(10**12, sieve_shape=7)

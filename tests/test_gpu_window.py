@@ -53,7 +53,7 @@ def test_windowed_partitions_schedules_and_shapes(engine, known):
         for sched in (mdr_hip.SCHED_DYNAMIC, mdr_hip.SCHED_STATIC):
             assert engine.mertens(X, opts(params=mdr_hip.PARAMS_TUNED, partition=part, sched=sched,
                                           window=3 * 10**8)) == known[X]
-    for shape in (1, 2, 3, 4):
+    for shape in (1, 2, 3, 4, 5, 6):
         print("windowed shape", shape, flush=True)
         assert engine.mertens(X, opts(params=mdr_hip.PARAMS_TUNED, sieve_shape=shape, window=2 * 10**8)) == known[X]
 
