@@ -53,20 +53,23 @@ class Tuner:
         self.expect = {}
 
     def time(self, X, cfg):
-        reps = self.reps if X < 10**17 else 1   # seconds-long calls: one run is stable
+        # GPU time of the call (table, partition, quotient phase: the bench's
+        # steady state); host-side allocation when the sweep alternates table
+        # sizes is left out.  The least of the repetitions.
+        reps = self.reps
         o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1, sched=mdr_hip.SCHED_DYNAMIC,
                               partition=mdr_hip.PARTITION_LEARNED, tune_u=cutoff(X, cfg["fx"], cfg["cx"]),
                               tune_ud=cfg["ud"], tune_cx=cfg["cx"], chunk=cfg["chunks"],
                               sieve_shape=cfg.get("sieve", 0) + 1)
         ts = []
         for _ in range(reps):
-            t0 = time.perf_counter()
             r = self.E.mertens(X, o)
-            ts.append(time.perf_counter() - t0)
+            st = self.E.last_stats
+            ts.append((st.table_ms + st.partition_ms + st.quotient_ms) / 1000.0)
             if X in self.expect and self.expect[X] != r:
                 raise SystemExit(f"M({X}) changed with {cfg}: {r} != {self.expect[X]}")
             self.expect[X] = r
-        return sorted(ts)[len(ts) // 2]
+        return min(ts)
 
     def cost(self, xs, cfg):
         return sum(self.time(X, cfg) for X in xs)

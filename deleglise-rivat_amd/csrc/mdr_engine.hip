@@ -57,9 +57,10 @@ struct DevBuf {
         cap = 0;
     }
     // ensure, after giving back an allocation far larger than needed (a big
-    // table must not hold the memory the next one needs)
+    // table must not hold the memory the next one needs; calls that alternate
+    // between nearby sizes keep their buffer)
     hipError_t fit(size_t bytes) {
-        if (p && cap > 2 * bytes && cap - bytes > ((size_t)1 << 30)) release();
+        if (p && cap > 2 * bytes && cap - bytes > ((size_t)64 << 30)) release();
         return ensure(bytes);
     }
     template <typename T> T* as() const { return static_cast<T*>(p); }
