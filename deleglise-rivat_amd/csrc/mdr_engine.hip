@@ -1171,7 +1171,8 @@ void mdr_shutdown(mdr_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     DevBuf* bufs[] = {&c->Md, &c->cl, &c->mu6, &c->segsum, &c->scan_tmp, &c->stencil, &c->wprimes,
                       &c->wrcp, &c->wweights, &c->sqprimes, &c->xparam, &c->slot_len, &c->slot_cost, &c->bounds,
-                      &c->out128, &c->ctr, &c->stamps, &c->comp_out, &c->lookup, &c->mlp};
+                      &c->out128, &c->ctr, &c->stamps, &c->comp_out, &c->lookup, &c->mlp,
+                      &c->wcl, &c->wseg, &c->wscan, &c->wcarry, &c->big};
     for (auto& g : c->graph_exec)
         if (g) (void)hipGraphExecDestroy(g);
     for (DevBuf* b : bufs) b->release();

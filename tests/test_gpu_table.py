@@ -116,3 +116,19 @@ def test_sieve_shapes_build_the_same_table(engine, mu_ref, pre_ref, known, shape
 def test_sieve_shape_out_of_range(engine):
     with pytest.raises(mdr_hip.MdrError):
         engine.mertens(10**12, sieve_shape=7)
+
+
+def test_segment_prefixes_across_a_large_table(engine, ref_values):
+    # the offset each segment gets from the scan of the segment sums: M at
+    # points spread over a 6e10 table against the compiled reference's values,
+    # and M differences against trial division across segment edges
+    U, ud = 60_000_000_000, 1048512
+    engine.table_build(U, ud, 10**6)
+    pts = [x for x in ref_values if x <= U][::9]
+    assert len(pts) > 20
+    for X in pts:
+        assert int(engine.read_M(X, 1)[0]) == ref_values[X], X
+    for k in [1, 2, 777, 123_457, U // SEG - 1]:
+        lo = k * SEG - 5000
+        M = engine.read_M(lo - 1, 10001).astype(np.int64)
+        assert np.array_equal(np.diff(M), segmented_mobius(lo, lo + 9999)), k
