@@ -195,17 +195,60 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
         // 2n <= 2^32, q <= U < 2^44.  q0 = nearest integer to the estimate
         // (2^52 + qd leaves it in the low mantissa bits) is q or q + 1, so the
         // true remainder y - q0*n lies in [-n, n) and its low 32 bits decide
-        // the correction exactly.  The line index off / 192 is taken in FP64:
-        // off is exact there (< 2^52) and (off + 1/2) / 192 is at least 1/384
-        // away from an integer, far above the rounding error.  Two positions
-        // per step share one reciprocal.
+        // the correction exactly; off = q - ud - 1 is the value's place in the
+        // compressed tier.  Two positions per step share one reciprocal.
         const uint32_t ylo = (uint32_t)y;
         const u64 base = (u64)tp.ud + 1;
-        const double cbase = 4503599627370496.0 + (double)base;   // 2^52 + base, exact
         const uint32_t iters = lane_iters(cnt, lane);
-        uint32_t n = (uint32_t)nlo + (uint32_t)(ST * lane);
+        const uint32_t n0 = (uint32_t)nlo + (uint32_t)(ST * lane);
         constexpr uint32_t D = 64u * ST;
-        auto lookup = [&](uint32_t nn, double rn) -> int32_t {
+        // int32 partial sums over at most 2^11 positions: the table holds
+        // q <= U < 192 * 2^32 < 2^40, where |M(q)| < 0.6 sqrt(q) < 2^19.3
+        // (|M(x)| / sqrt(x) stays below 0.6 through x = 10^16), so each
+        // partial sum stays below 2^30.3
+        auto run = [&](auto lookup) -> i64 {
+            uint32_t n = n0;
+            i64 acc = 0;
+            for (uint32_t k0 = 0; k0 < iters; k0 += 2048u) {
+                const uint32_t kend = iters - k0 < 2048u ? iters : k0 + 2048u;
+                int32_t part = 0;
+                uint32_t k = k0;
+                for (; k + 2 <= kend; k += 2) {
+                    double r1, r2;
+                    rcp_pair((double)n, (double)(n + D), r1, r2);
+                    part += lookup(n, r1);
+                    part += lookup(n + D, r2);
+                    n += 2 * D;
+                }
+                if (k < kend) {
+                    part += lookup(n, rcp_fast((double)n));
+                    n += D;
+                }
+                acc += part;
+            }
+            return acc;
+        };
+        if (tp.L <= 0x55555555ll) {
+            // fewer than 2^32 words (3 per line): the word index w = off >> 6
+            // fits 32 bits, so the line is w / 3 by a multiply-high.  The
+            // estimate's bit pattern is 0x433 << 52 | q0, so one 64-bit
+            // subtraction gives off = q0 - over - base.
+            const u64 cK = 0x4330000000000000ull + base;
+            return run([&](uint32_t nn, double rn) -> int32_t {
+                const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
+                const u64 bits = (u64)__double_as_longlong(qm);
+                const uint32_t r = ylo - (uint32_t)bits * nn;
+                const u64 off = bits - (cK + (r >= nn ? 1ull : 0ull));   // r >= nn: q0 = q + 1
+                const uint32_t w = (uint32_t)(off >> 6);
+                const uint32_t line = __umulhi(w, 0xaaaaaaabu) >> 1;
+                return line_M(tp.cl + line, w - 3u * line, (uint32_t)off & 63u);
+            });
+        }
+        // more words: the line index off / 192 is taken in FP64: off is
+        // exact there (< 2^52) and (off + 1/2) / 192 is at least 1/384 away
+        // from an integer, far above the rounding error
+        const double cbase = 4503599627370496.0 + (double)base;   // 2^52 + base, exact
+        return run([&](uint32_t nn, double rn) -> int32_t {
             const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
             const u64 q0 = (u64)__double_as_longlong(qm) & 0xfffffffffffffull;
             const uint32_t r = ylo - (uint32_t)q0 * nn;
@@ -216,29 +259,7 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
                 __builtin_fma(offd, 1.0 / LINE_VALUES, 0.5 / LINE_VALUES));
             const uint32_t sub = (uint32_t)(off >> 6) - 3u * line;
             return line_M(tp.cl + line, sub, (uint32_t)(off & 63));
-        };
-        // int32 partial sums over at most 2^11 positions: the table holds
-        // q <= U < 192 * 2^32 < 2^40, where |M(q)| < 0.6 sqrt(q) < 2^19.3
-        // (|M(x)| / sqrt(x) stays below 0.6 through x = 10^16), so each
-        // partial sum stays below 2^30.3
-        for (uint32_t k0 = 0; k0 < iters; k0 += 2048u) {
-            const uint32_t kend = iters - k0 < 2048u ? iters : k0 + 2048u;
-            int32_t part = 0;
-            uint32_t k = k0;
-            for (; k + 2 <= kend; k += 2) {
-                double r1, r2;
-                rcp_pair((double)n, (double)(n + D), r1, r2);
-                part += lookup(n, r1);
-                part += lookup(n + D, r2);
-                n += 2 * D;
-            }
-            if (k < kend) {
-                part += lookup(n, rcp_fast((double)n));
-                n += D;
-            }
-            s += part;
-        }
-        return s;
+        });
     }
     for (u64 i = lane; i < cnt; i += 64) {
         const u64 n = nlo + (u64)ST * i;
