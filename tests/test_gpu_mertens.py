@@ -139,6 +139,13 @@ def test_largest_table_and_external_1e18():
         o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=U, tune_ud=ud, no_table_reuse=1)
         assert e.mertens(X, o) == -46758740
         assert e.last_stats.U == U
+        # the compressed lookups take the line index in 32-bit integers up to
+        # 3 * 0x55555555 words (word indices up to 2^32 - 2) and in FP64 above
+        for lines in (0x55555555, 0x55555556):
+            U = ud + 192 * lines
+            o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=U, tune_ud=ud, no_table_reuse=1)
+            assert e.mertens(X, o) == -46758740, lines
+            assert e.last_stats.U == U
     finally:
         e.close()
 
