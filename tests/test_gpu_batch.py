@@ -83,3 +83,20 @@ def test_lookup_only_batches(engine, ref_values):
     out = engine.batch(xs, tuned())
     assert out == [ref_values[x] for x in xs]
     assert engine.last_stats.s1_terms == 0 and engine.last_stats.s2_terms == 0
+
+
+def test_large_mixed_batch_learned_equals_static(engine, xlists, ref_values):
+    # 64 X log-uniform in [1e16, 1e18] in one call: the learned partition and
+    # the static (equal-positions) partition give the same 64 values, and the
+    # values below 1.3e16 match the compiled reference
+    xs = xlists["large_64_1e16_1e18_seed5"]
+    print("large batch, learned partition", flush=True)
+    learned = engine.batch(xs, tuned(no_table_reuse=1, partition=mdr_hip.PARTITION_LEARNED))
+    assert engine.last_stats.partition_used == mdr_hip.PARTITION_LEARNED
+    print("large batch, static partition", flush=True)
+    static = engine.batch(xs, tuned(partition=mdr_hip.PARTITION_STATIC, sched=mdr_hip.SCHED_STATIC))
+    assert engine.last_stats.partition_used == mdr_hip.PARTITION_STATIC
+    assert learned == static
+    checked = [i for i, x in enumerate(xs) if x in ref_values]
+    assert len(checked) >= 4
+    assert all(learned[i] == ref_values[xs[i]] for i in checked)
