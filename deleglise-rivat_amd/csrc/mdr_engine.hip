@@ -294,10 +294,23 @@ void share_batch_cutoff(std::vector<XPlan>& plans, const mdr_opts& o) {
             umax = std::max(umax, p.u);
         }
     if (nsum < 2) return;
-    const i64 ub = tune_batch_u(xsum, o.tune_batch);
+    i64 ub = tune_batch_u(xsum, o.tune_batch);
+    // above the one-table cap the compressed tier is windowed, which needs
+    // every lookup inside the resident part: a batch with an X at or below
+    // the shared cutoff (it would become a lookup) keeps the capped cutoff
+    i64 window = 0;
+    if (ub > tune_u_cap()) {
+        bool inside = false;
+        for (const XPlan& p : plans)
+            if (p.kind == 2 && p.X <= ub) inside = true;
+        if (inside) ub = tune_u_cap() / 192 * 192;
+        else window = tune_u_cap();
+    }
     if (ub <= umax) return;
     for (XPlan& p : plans) {
-        if (p.kind != 2 || p.u >= ub) continue;
+        if (p.kind != 2) continue;
+        p.window = std::max(p.window, window);
+        if (p.u >= ub) continue;
         if (p.X <= ub) {
             p.kind = 1;
             p.u = p.X;

@@ -133,14 +133,15 @@ TuneChoice tune_lookup(i64 X, bool streaming) {
 // (linear in the cutoff) is paid once while the quotient work of every X
 // (proportional to X / sqrt(u)) falls as the cutoff grows: the optimum of
 // a U + sum_i b X_i / sqrt(U) is the single-X optimum at X = sum_i X_i.
-// The tuned batch factor (or factor > 0) scales it; the result is capped
-// like tune_lookup.
+// The tuned batch factor (or factor > 0) scales it; the result is capped at
+// 1e12 like tune_lookup (above the one-table cap the caller windows the
+// table).
 i64 tune_batch_u(double xsum, double factor) {
     std::call_once(g_once, load);
     const double xe = std::min(xsum, 9.0e18);
     const TuneChoice t = tune_lookup((i64)xe, true);
     double u = (double)t.u * (factor > 0 ? factor : g_batch_factor);
-    u = std::min(u, (double)g_u_cap.load());
+    u = std::min(u, (double)WINDOW_U_MAX);
     const i64 ui = ((i64)u / 192) * 192;
     return ui > 0 ? ui : 192;
 }

@@ -86,13 +86,15 @@ def test_lookup_only_batches(engine, ref_values):
 
 
 def test_large_mixed_batch_learned_equals_static(engine, xlists, ref_values):
-    # 64 X log-uniform in [1e16, 1e18] in one call: the learned partition and
-    # the static (equal-positions) partition give the same 64 values, and the
-    # values below 1.3e16 match the compiled reference
+    # 64 X log-uniform in [1e16, 1e18] in one call (a shared cutoff of 1e12,
+    # above the one-table cap, so the compressed tier is windowed): the
+    # learned partition and the static (equal-positions) partition give the
+    # same 64 values, and the values below 1.3e16 match the compiled reference
     xs = xlists["large_64_1e16_1e18_seed5"]
     print("large batch, learned partition", flush=True)
     learned = engine.batch(xs, tuned(no_table_reuse=1, partition=mdr_hip.PARTITION_LEARNED))
     assert engine.last_stats.partition_used == mdr_hip.PARTITION_LEARNED
+    assert engine.last_stats.windows >= 2 and engine.last_stats.U > 4 * 10**11
     print("large batch, static partition", flush=True)
     static = engine.batch(xs, tuned(partition=mdr_hip.PARTITION_STATIC, sched=mdr_hip.SCHED_STATIC))
     assert engine.last_stats.partition_used == mdr_hip.PARTITION_STATIC
@@ -100,3 +102,17 @@ def test_large_mixed_batch_learned_equals_static(engine, xlists, ref_values):
     checked = [i for i, x in enumerate(xs) if x in ref_values]
     assert len(checked) >= 4
     assert all(learned[i] == ref_values[xs[i]] for i in checked)
+
+
+def test_batch_with_an_x_below_the_shared_cutoff_stays_one_table(engine, ref_values):
+    # the shared cutoff of these X is above the one-table cap, but an X at or
+    # below it would become a lookup outside the resident part of a windowed
+    # table: the batch keeps one table at the capped cutoff
+    small = max(x for x in ref_values if x <= 2 * 10**11)
+    xs = [small, 10**18, 9 * 10**17 + 7]
+    out = engine.batch(xs, tuned(no_table_reuse=1))
+    st = engine.last_stats
+    assert st.windows == 0 and st.U <= 4 * 10**11
+    assert out[0] == ref_values[small]
+    assert out[1] == -46758740
+    assert out[2] == engine.mertens(xs[2], tuned(no_table_reuse=1))
