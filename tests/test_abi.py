@@ -110,8 +110,9 @@ def test_python_env_parser_follows_c_atof():
 
 
 @pytest.fixture(scope="module")
-def plan_probe():
-    exe = os.path.join(ROOT, "tests", "cpp", "plan_probe")
+def plan_probe(tmp_path_factory):
+    # built per test process (parallel runners must not overwrite a running copy)
+    exe = str(tmp_path_factory.mktemp("plan_probe") / "plan_probe")
     subprocess.run(["g++", "-O1", "-std=c++20", "-I" + os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", "plan_probe.cpp"), "-L" + os.path.join(ROOT, "deleglise-rivat_amd"),
                     "-lmdr_hip", "-Wl,-rpath," + os.path.join(ROOT, "deleglise-rivat_amd"), "-o", exe], check=True)
