@@ -15,10 +15,15 @@
 // MERTENS_FX, MERTENS_U_CAP.  Errors raise std::runtime_error (the reference's
 // CUDA engine does the same through CUDA_CHECK, mertens_cuda.cuh:20-28).
 //
+// Parameters: with none of those five variables set, a call takes the GPU's
+// autotuned cutoff and split (the result is the same integer; M(X) does not
+// depend on them).  Setting any of them selects the reference's parameter
+// rules with that control applied, as the reference would use it.
+//
 // Extra controls of the GPU engine (none changes a result):
 //   MERTENS_HIP_DEVICE   GPU ordinal (default 0)
-//   MERTENS_HIP_TUNED=1  use the autotuned GPU cutoff/split instead of the
-//                        reference parameters
+//   MERTENS_HIP_TUNED=1  the autotuned cutoff/split even with reference
+//                        controls set; =0 the reference's rules always
 //   MERTENS_HIP_LEARNED=1  learned (MLP) chunk partition instead of static
 //
 // num_threads is accepted for interface parity; the GPU engine does not use
@@ -57,6 +62,10 @@ namespace hip_detail {
 inline mdr_opts opts_from_env(int strategy_override = -1) {
     mdr_opts o;
     mdr_default_opts(&o);
+    bool reference_control = false;
+    for (const char* v : {"MERTENS_USE_CHMT", "MERTENS_CX", "MERTENS_CHMT_U_CAP", "MERTENS_FX", "MERTENS_U_CAP"})
+        if (std::getenv(v)) reference_control = true;
+    o.params = reference_control ? MDR_PARAMS_REFERENCE : MDR_PARAMS_TUNED;
     if (const char* e = std::getenv("MERTENS_USE_CHMT")) {
         const bool on = e[0] == '1' || e[0] == 'y' || e[0] == 'Y' || e[0] == 't' || e[0] == 'T';
         o.strategy = on ? MDR_STRATEGY_STREAMING : MDR_STRATEGY_DENSE;

@@ -35,6 +35,9 @@ static inline int64 compute_mertens_windowed(int64 X, int num_threads = 0, int64
     if (X < 1) return 0;
     mdr_ctx* c = mertens_dr::hip_detail::context().get();
     mdr_opts o = mertens_dr::hip_detail::opts_from_env(MDR_STRATEGY_DENSE);
+    // the windowed engine keeps the reference's dense-strategy cutoff unless
+    // MERTENS_HIP_TUNED=1 asks for the tuned one
+    if (!std::getenv("MERTENS_HIP_TUNED")) o.params = MDR_PARAMS_REFERENCE;
     o.window = std::max(block_M, MIN_GPU_WINDOW);
     mdr_stats st;
     long long out = 0;

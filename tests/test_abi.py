@@ -141,6 +141,18 @@ def test_dropin_header_parameter_helpers(plan_probe, env):
         chmt = env.get("MERTENS_USE_CHMT")
         assert int(f[5]) == (1 if chmt and chmt[:1] in "1yYtT" else 0)
         assert int(f[4]) == (1 if (X >= 10**11 and "MERTENS_FX" not in env and "MERTENS_U_CAP" not in env) else 0)
+        # no reference control set: the tuned GPU plan; any set: the reference's rules
+        assert int(f[7]) == (mdr_hip.PARAMS_TUNED if not env else mdr_hip.PARAMS_REFERENCE), env
+
+
+def test_dropin_tuned_switch_overrides_the_default(plan_probe):
+    base = {k: v for k, v in os.environ.items() if not k.startswith("MERTENS_")}
+    for extra, want in [({"MERTENS_HIP_TUNED": "0"}, mdr_hip.PARAMS_REFERENCE),
+                        ({"MERTENS_HIP_TUNED": "1", "MERTENS_CX": "1.7"}, mdr_hip.PARAMS_TUNED),
+                        ({"MERTENS_HIP_TUNED": "1"}, mdr_hip.PARAMS_TUNED)]:
+        out = subprocess.run([plan_probe, str(10**17)], capture_output=True, text=True, env=dict(base, **extra),
+                             check=True).stdout.split()
+        assert int(out[7]) == want, extra
 
 
 def test_cli_usage_and_no_device():
