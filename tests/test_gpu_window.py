@@ -118,3 +118,17 @@ def test_windowed_call_runs_eagerly_and_logs_cycles(engine, known):
 def test_negative_window_is_rejected(engine):
     with pytest.raises(mdr_hip.MdrError):
         engine.mertens(10**12, opts(window=-1))
+
+
+def test_windowed_batch_with_small_and_constant_x(engine):
+    # a tuned batch whose shared cutoff exceeds the one-table cap is windowed;
+    # its constant (X <= 3) and read-from-table (X <= 5e7) members live in the
+    # resident part
+    import oracle
+    xs = [3, 10**7 + 19, 10**18, 9 * 10**17 + 7]
+    out = engine.batch(xs, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1))
+    assert engine.last_stats.windows >= 1
+    assert out[0] == -1
+    assert out[1] == oracle.mertens(xs[1], 8)
+    assert out[2] == -46758740
+    assert out[3] == engine.mertens(xs[3], mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1))
