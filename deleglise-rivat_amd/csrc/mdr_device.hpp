@@ -147,9 +147,8 @@ __device__ __forceinline__ int mu6_mu(const uint32_t* mu6, i64 t) {
 // The prefix popcount over bits 0..b is the popcount of the word shifted left
 // by 63 - b (the bits above b leave the word), which needs no mask.
 __device__ __forceinline__ int32_t line_M(const CLine* L, uint32_t sub, uint32_t b) {
-    const char* la = reinterpret_cast<const char*>(L);
-    const uint4 pw = *reinterpret_cast<const uint4*>(la + (sub << 4));
-    const int32_t ck = *reinterpret_cast<const int32_t*>(la + 48 + (sub << 2));
+    const uint4 pw = reinterpret_cast<const uint4*>(L)[sub];          // planes at 16 sub
+    const int32_t ck = reinterpret_cast<const int32_t*>(L)[12u + sub];  // checkpoint at 48 + 4 sub
     const uint32_t sh = 63u - (b & 63u);
     const u64 nz = ((u64)pw.y << 32 | pw.x) << sh;
     const u64 ng = ((u64)pw.w << 32 | pw.z) << sh;

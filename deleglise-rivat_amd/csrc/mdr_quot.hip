@@ -228,17 +228,20 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
             }
             return acc;
         };
-        if (tp.L <= 0x55555555ll) {
+        const u64 cK = 0x4330000000000000ull + base;
+        if (tp.L <= 0x55555555ll && (uint32_t)cK != 0xffffffffu) {
             // fewer than 2^32 words (3 per line): the word index w = off >> 6
             // fits 32 bits, so the line is w / 3 by a multiply-high.  The
             // estimate's bit pattern is 0x433 << 52 | q0, so one 64-bit
-            // subtraction gives off = q0 - over - base.
-            const u64 cK = 0x4330000000000000ull + base;
+            // subtraction gives off = q0 - over - base (over folded into
+            // the low word of the subtrahend, which cannot carry).
+            const uint32_t cK_lo = (uint32_t)cK, cK_hi = (uint32_t)(cK >> 32);
             return run([&](uint32_t nn, double rn) -> int32_t {
                 const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
                 const u64 bits = (u64)__double_as_longlong(qm);
                 const uint32_t r = ylo - (uint32_t)bits * nn;
-                const u64 off = bits - (cK + (r >= nn ? 1ull : 0ull));   // r >= nn: q0 = q + 1
+                const uint32_t sub_lo = cK_lo + (r >= nn ? 1u : 0u);   // r >= nn: q0 = q + 1
+                const u64 off = bits - ((u64)cK_hi << 32 | sub_lo);
                 const uint32_t w = (uint32_t)(off >> 6);
                 const uint32_t line = __umulhi(w, 0xaaaaaaabu) >> 1;
                 return line_M(tp.cl + line, w - 3u * line, (uint32_t)off & 63u);
