@@ -135,9 +135,9 @@ __host__ __device__ __forceinline__ i64 count_coprime6(i64 x) {
 __host__ __device__ __forceinline__ i64 j_of_t(i64 t) { return 3 * t + 1 + (t & 1); }
 
 // mu(j(t)) from its 2-bit two's-complement code (1 = 01, 0 = 00, -1 = 11):
-// a signed bit-field extract.
+// a signed bit-field extract (one v_bfe_i32; shift is even, at most 30).
 __device__ __forceinline__ int mu6_field(uint32_t word, uint32_t shift) {
-    return (int)(word << (30u - shift)) >> 30;
+    return __builtin_amdgcn_sbfe((int)word, shift, 2u);
 }
 __device__ __forceinline__ int mu6_mu(const uint32_t* mu6, i64 t) {
     return mu6_field(mu6[t >> 4], (uint32_t)(2 * (t & 15)));
