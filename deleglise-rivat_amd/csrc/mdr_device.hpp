@@ -41,6 +41,22 @@ __device__ __forceinline__ void rcp_pair(double a, double b, double& ra, double&
     rb = r * a;
 }
 
+// Four reciprocals from one v_rcp_f64: 1/(a b c d) refined once, times c d
+// and a b, then times the other factor of each pair.  Each result is within
+// 2^-48 relative (the three products round by 2^-53 each, the refined
+// reciprocal is within 2^-48.6, the two further products round by 2^-53
+// each); a, b, c, d < 2^32, so a b c d < 2^128.
+__device__ __forceinline__ void rcp_quad(double a, double b, double c, double d, double& ra, double& rb,
+                                         double& rc, double& rd) {
+    const double ab = a * b, cd = c * d;
+    const double r = rcp_fast(ab * cd);
+    const double rab = r * cd, rcd = r * ab;
+    ra = rab * b;
+    rb = rab * a;
+    rc = rcd * d;
+    rd = rcd * c;
+}
+
 // q = floor(y/n) for q <= 2^32 - 2 and 2n <= 2^32, from an estimate rn of 1/n
 // within 2^-47 relative.  2^52 + y*rn (one FMA) rounds the estimate, which is
 // within 2^-15 of y/n, to the nearest integer q0 in {q, q + 1} and leaves q0 in

@@ -311,7 +311,7 @@ __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane)
         s += (i64)mu6_mu(tp.mu6, (i64)t) * piece_eval<P, u64>(q);
     }
     // A lane's t advances by 64, so j by 192, its mu word by 4 and the
-    // field's shift stays; two positions per step share one reciprocal.
+    // field's shift stays; four positions per step share one reciprocal.
     const uint32_t iters = lane_iters(t_hi - ts, lane);
     if (iters) {
         const uint32_t ylo = (uint32_t)y;
@@ -320,7 +320,21 @@ __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane)
         const uint32_t sh = 2u * (t0 & 15u);
         uint32_t j = 3u * t0 + 1u + (t0 & 1u);
         uint32_t k = 0;
-        for (; k + 2 <= iters; k += 2) {
+        for (; k + 4 <= iters; k += 4) {
+            double r1, r2, r3, r4;
+            rcp_quad((double)j, (double)(j + 192u), (double)(j + 384u), (double)(j + 576u), r1, r2, r3, r4);
+            const uint32_t q1 = qdiv32n(ylo, dy, j, r1);
+            const uint32_t q2 = qdiv32n(ylo, dy, j + 192u, r2);
+            const uint32_t q3 = qdiv32n(ylo, dy, j + 384u, r3);
+            const uint32_t q4 = qdiv32n(ylo, dy, j + 576u, r4);
+            s += (i64)mu6_field(wp[0], sh) * (i64)(int32_t)piece32<P>(q1);
+            s += (i64)mu6_field(wp[4], sh) * (i64)(int32_t)piece32<P>(q2);
+            s += (i64)mu6_field(wp[8], sh) * (i64)(int32_t)piece32<P>(q3);
+            s += (i64)mu6_field(wp[12], sh) * (i64)(int32_t)piece32<P>(q4);
+            j += 768u;
+            wp += 16;
+        }
+        if (k + 2 <= iters) {
             double r1, r2;
             rcp_pair((double)j, (double)(j + 192u), r1, r2);
             const uint32_t q1 = qdiv32n(ylo, dy, j, r1);
@@ -329,6 +343,7 @@ __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane)
             s += (i64)mu6_field(wp[4], sh) * (i64)(int32_t)piece32<P>(q2);
             j += 384u;
             wp += 8;
+            k += 2;
         }
         if (k < iters)
             s += (i64)mu6_field(wp[0], sh) * (i64)(int32_t)piece32<P>(qdiv32n(ylo, dy, j, rcp_fast((double)j)));
