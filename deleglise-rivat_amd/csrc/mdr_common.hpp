@@ -33,6 +33,17 @@ constexpr int LINE_VALUES = 192;
 // Largest compressed tier (U - u_dense): line indices stay below 2^32.
 constexpr long long MAX_COMPRESSED_VALUES = (long long)LINE_VALUES * 4294967295LL;
 
+// Sieve workgroup shapes (segment length, threads): the table they build is
+// the same; the autotuner picks one per decade of X.  Segment lengths are
+// 3 * 2^shift, multiples of 192 (lines), 48 (mu words) and 16.
+struct SieveShape {
+    int seg, threads, shift;
+};
+constexpr SieveShape SIEVE_SHAPES[] = {{49152, 1024, 14}, {24576, 256, 13}, {98304, 1024, 15},
+                                       {49152, 256, 14},  {49152, 512, 14},  {24576, 512, 13}};
+constexpr int N_SIEVE_SHAPES = (int)(sizeof(SIEVE_SHAPES) / sizeof(SIEVE_SHAPES[0]));
+constexpr int SEG_MAX = 98304;
+
 struct alignas(64) CLine {
     u64 pl[6];        // word w: pl[2w] = nonzero plane, pl[2w+1] = negative plane
     int32_t ck[3];    // word w: M(value before its first value)

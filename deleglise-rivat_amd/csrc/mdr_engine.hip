@@ -66,6 +66,36 @@ struct DevBuf {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// One device-resident table: M(n) for n <= U (int16 dense tier up to ud,
+// 64-byte compressed lines above), packed mu codes up to mu, and the segment
+// sums of its build.
+struct TableStore {
+    bool valid = false;
+    i64 U = 0, ud = 0, mu = 0, L = 0, nseg = 0, musegs = 0;
+    DevBuf Md, cl, mu6, segsum;
+    TableParam tp() const {
+        TableParam t;
+        t.U = U;
+        t.ud = ud;
+        t.L = L;
+        t.mu_lim = mu;
+        t.Md = Md.as<int16_t>();
+        t.cl = cl.as<CLine>();
+        t.mu6 = mu6.as<uint32_t>();
+        t.pass = 0;
+        t.qlo = ud + 1;
+        t.qhi = U;
+        return t;
+    }
+    void release() {
+        Md.release();
+        cl.release();
+        mu6.release();
+        segsum.release();
+        valid = false;
+    }
+};
+
 i64 isqrt_host(i64 n) {
     if (n <= 0) return 0;
     i64 s = (i64)std::sqrt((double)n);
@@ -86,10 +116,9 @@ struct mdr_ctx {
     std::string err;
     hipEvent_t ev[8] = {};
 
-    // table
-    bool tab_valid = false;
-    i64 tU = 0, tud = 0, tmu = 0, tL = 0, tnseg = 0, tmusegs = 0;
-    DevBuf Md, cl, mu6, segsum, scan_tmp;
+    // the cached table (reused by later calls that it covers)
+    TableStore tab;
+    DevBuf scan_tmp;
     // sieving primes
     i64 primes_root = -1;
     int n_w = 0, n_small = 0, n_sq = 0, n_sq_small = 0;
@@ -100,7 +129,8 @@ struct mdr_ctx {
     // Dirichlet cross-check engine: M(x/k) for the large quotients
     DevBuf big;
     // work
-    DevBuf xparam, slot_len, slot_cost, bounds, out128, ctr, stamps, comp_out, lookup;
+    DevBuf xparam, slot_len, slot_cost, bounds, out128, ctr, stamps, comp_out, lookup, wspan;
+    int wall_khz = 0;   // rate of the wall clock the quotient waves stamp
     std::vector<StampRec> last_stamps;
     // instantiated graphs, one per launch sequence (see launch_graph)
     hipGraphExec_t graph_exec[16] = {};
@@ -109,20 +139,6 @@ struct mdr_ctx {
     bool mlp_ok = false;
     std::vector<float> mlp_host;
 
-    TableParam tp() const {
-        TableParam t;
-        t.U = tU;
-        t.ud = tud;
-        t.L = tL;
-        t.mu_lim = tmu;
-        t.Md = Md.as<int16_t>();
-        t.cl = cl.as<CLine>();
-        t.mu6 = mu6.as<uint32_t>();
-        t.pass = 0;
-        t.qlo = tud + 1;
-        t.qhi = tU;
-        return t;
-    }
 };
 
 namespace {
@@ -226,6 +242,7 @@ mdr_status validate_opts(mdr_ctx* c, const mdr_opts& o) {
     if (o.tune_batch != 0 && !(o.tune_batch >= 0.05 && o.tune_batch <= 20))
         return fail(c, MDR_ERR_INVALID, "tune_batch outside [0.05, 20]");
     if (o.window < 0) return fail(c, MDR_ERR_INVALID, "window must be >= 0");
+    if (o.wave_span < 0 || o.wave_span > 1) return fail(c, MDR_ERR_INVALID, "wave_span must be 0 or 1");
     return MDR_OK;
 }
 
@@ -380,6 +397,7 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
 // ---- table -----------------------------------------------------------------------
 // A table request after validation: the sizes every kernel of the build sees.
 struct TableJob {
+    TableStore* t = nullptr;   // the table being built
     bool build = false;   // false: the cached table already covers the request
     int shape = 0;        // sieve workgroup shape (SIEVE_SHAPES)
     i64 U = 0, ud = 0, mu_lim = 0, NL = 0, nseg = 0, musegs = 0;
@@ -387,7 +405,8 @@ struct TableJob {
 
 // Validates a table request and allocates what its build needs (host side:
 // everything that may allocate happens here, before any kernel is enqueued).
-mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, int shape, TableJob& j) {
+mdr_status prepare_table(mdr_ctx* c, TableStore& t, i64 U, i64 ud, i64 mu_lim, bool reuse, int shape,
+                         TableJob& j) {
     if (U < 1) U = 1;
     if (ud > U) ud = U;
     if (ud < U && (ud % LINE_VALUES) != 0)
@@ -399,6 +418,7 @@ mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, int 
     if (shape < 0 || shape >= N_SIEVE_SHAPES) return fail(c, MDR_ERR_INVALID, "sieve shape out of range");
     const i64 seg = SIEVE_SHAPES[shape].seg;
     j = TableJob{};
+    j.t = &t;
     j.shape = shape;
     j.U = U;
     j.ud = ud;
@@ -406,24 +426,24 @@ mdr_status prepare_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, bool reuse, int 
     j.nseg = (U + seg - 1) / seg;
     j.musegs = (mu_lim + seg - 1) / seg;
     j.NL = U > ud ? (U - ud + LINE_VALUES - 1) / LINE_VALUES : 0;
-    if (reuse && c->tab_valid && c->tU == U && c->tud == ud && c->tmu == mu_lim) return MDR_OK;
+    if (reuse && t.valid && t.U == U && t.ud == ud && t.mu == mu_lim) return MDR_OK;
     j.build = true;
-    c->tab_valid = false;
+    t.valid = false;
     mdr_status s = ensure_primes(c, U);
     if (s != MDR_OK) return s;
-    HIPT(c->Md.fit((size_t)(ud + 1) * 2), "alloc dense table");
-    HIPT(c->cl.fit((size_t)j.NL * sizeof(CLine)), "alloc compressed lines");
-    HIPT(c->mu6.ensure((size_t)j.musegs * (seg / 48) * 4), "alloc mu");
-    HIPT(c->segsum.ensure((size_t)j.nseg * 8), "alloc segment sums");
+    HIPT(t.Md.fit((size_t)(ud + 1) * 2), "alloc dense table");
+    HIPT(t.cl.fit((size_t)j.NL * sizeof(CLine)), "alloc compressed lines");
+    HIPT(t.mu6.ensure((size_t)j.musegs * (seg / 48) * 4), "alloc mu");
+    HIPT(t.segsum.ensure((size_t)j.nseg * 8), "alloc segment sums");
     HIPT(c->scan_tmp.ensure((size_t)scan_tmp_len(j.nseg) * 8), "alloc scan");
     // the quotient kernels read the table through these from now on; the
     // table counts as valid only once its build has completed
-    c->tU = U;
-    c->tud = ud;
-    c->tmu = mu_lim;
-    c->tL = j.NL;
-    c->tnseg = j.nseg;
-    c->tmusegs = j.musegs;
+    t.U = U;
+    t.ud = ud;
+    t.mu = mu_lim;
+    t.L = j.NL;
+    t.nseg = j.nseg;
+    t.musegs = j.musegs;
     return MDR_OK;
 }
 
@@ -443,10 +463,10 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     a.sqprimes = c->sqprimes.as<uint32_t>();
     a.n_sq = c->n_sq;
     a.n_sq_small = c->n_sq_small;
-    a.Md = c->Md.as<int16_t>();
-    a.cl = c->cl.as<CLine>();
-    a.mu6 = c->mu6.as<uint32_t>();
-    a.segsum = c->segsum.as<i64>();
+    a.Md = j.t->Md.as<int16_t>();
+    a.cl = j.t->cl.as<CLine>();
+    a.mu6 = j.t->mu6.as<uint32_t>();
+    a.segsum = j.t->segsum.as<i64>();
     a.seg0 = 0;
     // a dispatch holds fewer than 2^32 work-items: at most 2^31 threads per launch
     const i64 maxgrid = ((i64)1 << 31) / SIEVE_SHAPES[j.shape].threads;
@@ -455,7 +475,7 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
         const i64 nb = std::min(maxgrid, j.nseg - s0);
         HIPT(launch_sieve(j.shape, (unsigned)nb, c->st, a), "launch sieve");
     }
-    HIPT(scan_exclusive_i64(c->segsum.as<i64>(), c->segsum.as<i64>(), j.nseg, c->scan_tmp.as<i64>(),
+    HIPT(scan_exclusive_i64(j.t->segsum.as<i64>(), j.t->segsum.as<i64>(), j.nseg, c->scan_tmp.as<i64>(),
                             scan_tmp_len(j.nseg), nullptr, c->st),
          "scan segments");
     FixupArgs f;
@@ -463,9 +483,9 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     f.L = j.NL;
     f.nd = j.ud;
     f.seg_shift = SIEVE_SHAPES[j.shape].shift;
-    f.Md = c->Md.as<int16_t>();
-    f.cl = c->cl.as<CLine>();
-    f.segoff = c->segsum.as<i64>();
+    f.Md = j.t->Md.as<int16_t>();
+    f.cl = j.t->cl.as<CLine>();
+    f.segoff = j.t->segsum.as<i64>();
     f.seg0 = 0;
     f.carry = nullptr;
     const i64 nfix = std::max<i64>(1, std::max(j.ud, j.NL));
@@ -480,10 +500,11 @@ double table_bytes(const TableJob& j) {
            (double)j.musegs * (SIEVE_SHAPES[j.shape].seg / 12);
 }
 
-// Builds (or reuses) the table and waits for it.
-mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, bool reuse = true, int shape = 0) {
+// Builds (or reuses) a table and waits for it.
+mdr_status build_table(mdr_ctx* c, TableStore& t, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, bool reuse = true,
+                       int shape = 0) {
     TableJob j;
-    mdr_status s = prepare_table(c, U, ud, mu_lim, reuse, shape, j);
+    mdr_status s = prepare_table(c, t, U, ud, mu_lim, reuse, shape, j);
     if (s != MDR_OK) return s;
     if (!j.build) {
         if (stats) stats->table_reused = 1;
@@ -494,7 +515,7 @@ mdr_status build_table(mdr_ctx* c, i64 U, i64 ud, i64 mu_lim, mdr_stats* stats, 
     if (s != MDR_OK) return s;
     HIPT(hipEventRecord(c->ev[1], c->st), "event");
     HIPT(hipEventSynchronize(c->ev[1]), "table build");
-    c->tab_valid = true;
+    t.valid = true;
     if (stats) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
@@ -650,9 +671,9 @@ mdr_status enqueue_window_table(mdr_ctx* c, const WindowPlan& w, i64 L, i64 R, c
     a.sqprimes = c->sqprimes.as<uint32_t>();
     a.n_sq = c->n_sq;
     a.n_sq_small = c->n_sq_small;
-    a.Md = c->Md.as<int16_t>();
+    a.Md = c->tab.Md.as<int16_t>();
     a.cl = c->wcl.as<CLine>();
-    a.mu6 = c->mu6.as<uint32_t>();
+    a.mu6 = c->tab.mu6.as<uint32_t>();
     a.segsum = c->wseg.as<i64>();
     a.seg0 = s0;
     const i64 maxgrid = ((i64)1 << 31) / sh.threads;
@@ -669,7 +690,7 @@ mdr_status enqueue_window_table(mdr_ctx* c, const WindowPlan& w, i64 L, i64 R, c
     f.L = NL;
     f.nd = 0;
     f.seg_shift = sh.shift;
-    f.Md = c->Md.as<int16_t>();
+    f.Md = c->tab.Md.as<int16_t>();
     f.cl = c->wcl.as<CLine>();
     f.segoff = c->wseg.as<i64>();
     f.seg0 = s0;
@@ -777,30 +798,30 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     const bool windowed = win.count() > 0;
     // tuned calls: a cached table that covers the request serves it, and every
     // summed X takes the cached cutoff (M(X) does not depend on u)
-    if (need_table && !windowed && !o.no_table_reuse && c->tab_valid && o.params == MDR_PARAMS_TUNED &&
-        o.tune_u == 0 && c->tU >= U && c->tmu >= mu_lim) {
+    if (need_table && !windowed && !o.no_table_reuse && c->tab.valid && o.params == MDR_PARAMS_TUNED &&
+        o.tune_u == 0 && c->tab.U >= U && c->tab.mu >= mu_lim) {
         for (XPlan& p : plans) {
-            if (p.kind != 2 || p.u >= c->tU) continue;
-            if (p.X <= c->tU) {
+            if (p.kind != 2 || p.u >= c->tab.U) continue;
+            if (p.X <= c->tab.U) {
                 p.kind = 1;
                 p.u = p.X;
                 p.mu_lim = 1;
                 continue;
             }
-            p.u = c->tU;
+            p.u = c->tab.U;
             p.N = p.X / p.u;
             const i64 A2 = (i64)(p.cx * std::sqrt((double)p.X)) + 1000;
             p.mu_lim = std::max(A2, p.N + 1000);
         }
-        U = c->tU;
-        ud = c->tud;
-        mu_lim = c->tmu;
+        U = c->tab.U;
+        ud = c->tab.ud;
+        mu_lim = c->tab.mu;
     }
     BatchLaunch b;
     b.nx = nx;
     if (need_table) {
         if (!windowed) c->wcl.release();   // a one-table call needs the memory
-        mdr_status s = prepare_table(c, U, ud, mu_lim, !o.no_table_reuse, shape, b.tj);
+        mdr_status s = prepare_table(c, c->tab, U, ud, mu_lim, !o.no_table_reuse, shape, b.tj);
         if (s != MDR_OK) return s;
         if (stats && !b.tj.build) stats->table_reused = 1;
         if (windowed) {
@@ -808,7 +829,7 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
             if (s != MDR_OK) return s;
         }
     }
-    b.tp = c->tp();
+    b.tp = c->tab.tp();
 
     std::vector<XParam> xps;
     std::vector<int> xidx;
@@ -903,6 +924,11 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
         qa.stamp_cap = b.cap;
         qa.stamp_stride = o.record_cycles > 1 ? o.record_cycles : 1;
         qa.out128 = c->comp_out.as<u64>();
+        qa.wave_span = nullptr;
+        if (o.wave_span) {
+            HIPT(c->wspan.ensure((size_t)nwaves * 16), "alloc wave stamps");
+            qa.wave_span = c->wspan.as<u64>();
+        }
     }
     if (stats) stats->h2d_copy_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
 
@@ -923,7 +949,7 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
         if (s != MDR_OK) return s;
     }
     HIPT(hipStreamSynchronize(c->st), "batch");
-    if (b.tj.build) c->tab_valid = true;
+    if (b.tj.build) c->tab.valid = true;
     if (b.learned && windowed) {   // the resident pass's predicted total
         long long pc = 0;
         HIPT(d2h(c, &pc, c->ctr.as<unsigned long long>() + 5, 8), "download");
@@ -992,6 +1018,23 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
         if (hi != (lo >> 63)) return fail(c, MDR_ERR_RANGE, "result does not fit in 64 bits");
         out[i] = lo;
     }
+    if (b.nxs && o.wave_span && stats) {
+        // every resident wave stamps once per launch (last launch of the call)
+        const i64 nwaves = (i64)c->ncu * c->quot_blocks_per_cu * 4;
+        std::vector<u64> sp((size_t)nwaves * 2);
+        HIPT(d2h(c, sp.data(), c->wspan.p, sp.size() * 8), "download wave stamps");
+        u64 t0 = ~0ull, t1 = 0;
+        double busy = 0;
+        for (i64 w = 0; w < nwaves; ++w) {
+            t0 = std::min(t0, sp[2 * w]);
+            t1 = std::max(t1, sp[2 * w + 1]);
+            busy += (double)(sp[2 * w + 1] - sp[2 * w]);
+        }
+        const double khz = c->wall_khz > 0 ? (double)c->wall_khz : 100000.0;
+        stats->wave_span_ms = (double)(t1 - t0) / khz;
+        stats->wave_busy_ms = busy / (double)nwaves / khz;
+        stats->wave_busy_frac = t1 > t0 ? busy / (double)nwaves / (double)(t1 - t0) : 0.0;
+    }
     if (b.cap) {
         const i64 n = std::min<i64>((i64)ch[6], b.cap);
         c->last_stamps.resize((size_t)n);
@@ -1026,10 +1069,10 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
                                     : o.partition == MDR_PARTITION_TERMS ? MDR_PARTITION_TERMS
                                                                          : MDR_PARTITION_STATIC;
         }
-        stats->U = windowed ? win.U : c->tU;
-        stats->u_dense = c->tud;
-        stats->mu_limit = c->tmu;
-        stats->W = c->tL;
+        stats->U = windowed ? win.U : c->tab.U;
+        stats->u_dense = c->tab.ud;
+        stats->mu_limit = c->tab.mu;
+        stats->W = c->tab.L;
         if (!plans.empty()) {
             const XPlan& p = plans.back();
             stats->u = p.u;
@@ -1095,9 +1138,9 @@ mdr_status dirichlet_mertens(mdr_ctx* c, i64 X, long long* out, mdr_stats* stats
     L = std::min<i64>({L, tune_u_cap(), X});
     L = std::max<i64>(L, std::min<i64>(X, r + 1));
     const i64 ud = L <= 1048512 ? L : 1048512;
-    mdr_status s = build_table(c, L, ud, 1, stats);
+    mdr_status s = build_table(c, c->tab, L, ud, 1, stats);
     if (s != MDR_OK) return s;
-    const TableParam tp = c->tp();
+    const TableParam tp = c->tab.tp();
     const i64 K = X / (L + 1);
     HIPT(c->big.ensure((size_t)(K + 2) * 8), "alloc large values");
     i64* big = c->big.as<i64>();
@@ -1152,6 +1195,7 @@ mdr_status mdr_init(mdr_ctx** out, int device) {
     for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->clock_khz, hipDeviceAttributeClockRate, device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device);
     int occ = 0;
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_quot, 256, 0);
     if (e != hipSuccess) {
@@ -1182,10 +1226,10 @@ void mdr_shutdown(mdr_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
-    DevBuf* bufs[] = {&c->Md, &c->cl, &c->mu6, &c->segsum, &c->scan_tmp, &c->stencil, &c->wprimes,
+    DevBuf* bufs[] = {&c->tab.Md, &c->tab.cl, &c->tab.mu6, &c->tab.segsum, &c->scan_tmp, &c->stencil, &c->wprimes,
                       &c->wrcp, &c->wweights, &c->sqprimes, &c->xparam, &c->slot_len, &c->slot_cost, &c->bounds,
                       &c->out128, &c->ctr, &c->stamps, &c->comp_out, &c->lookup, &c->mlp,
-                      &c->wcl, &c->wseg, &c->wscan, &c->wcarry, &c->big};
+                      &c->wcl, &c->wseg, &c->wscan, &c->wcarry, &c->big, &c->wspan};
     for (auto& g : c->graph_exec)
         if (g) (void)hipGraphExecDestroy(g);
     for (DevBuf* b : bufs) b->release();
@@ -1262,7 +1306,7 @@ mdr_status mdr_table_build(mdr_ctx* c, long long U, long long u_dense, long long
     std::lock_guard<std::mutex> lk(c->mu);
     c->err.clear();
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
-    return build_table(c, U, u_dense, mu_limit, nullptr);
+    return build_table(c, c->tab, U, u_dense, mu_limit, nullptr);
 }
 
 }  // extern "C"
@@ -1343,12 +1387,59 @@ __global__ void k_component(TableParam tp, Seg sg, u64 y, int geval, int comb2, 
     if (lane == 0) atomic_add128(out128, lo, hi);
 }
 
+// out[i] = M(q[i]) (gathers for table objects)
+__global__ void k_gather_M(TableParam tp, const i64* q, i64 n, int32_t* out) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = get_M(tp, q[i]);
+}
+
+// The compressed tier in the reference's layout (mertens_dr.hpp:670-687,
+// :741-752) for a dense limit ref_ud at or above this table's own (ud): one
+// thread per 64-value word w of chunk c0 + w / 4, covering n = ref_ud + 1 +
+// 64 w' ...; the word's bits are this table's plane bits at offset
+// n - ud - 1 (one or two of its words), its checkpoint M(n - 1).  Words past
+// lim read as zero with a zero checkpoint.
+__global__ void k_export_chunks(TableParam tp, i64 lim, i64 ref_ud, i64 c0, i64 nwords, u64* chunks, int32_t* ck) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nwords) return;
+    const i64 w = 4 * c0 + t;                  // word of the reference tier
+    const i64 n0 = ref_ud + 1 + 64 * w;
+    u64 nz = 0, ng = 0;
+    int32_t m0 = 0;
+    if (n0 <= lim) {
+        m0 = get_M(tp, n0 - 1);
+        const u64 d = (u64)(n0 - tp.ud - 1);   // >= 0: ref_ud >= ud
+        const u64 gw = d >> 6;
+        const uint32_t b = (uint32_t)(d & 63);
+        const CLine* L0 = tp.cl + gw / 3;
+        nz = L0->pl[2 * (gw % 3)] >> b;
+        ng = L0->pl[2 * (gw % 3) + 1] >> b;
+        // the next word holds values only when it starts at or below U
+        if (b && (i64)(64 * (gw + 1)) < tp.U - tp.ud) {
+            const CLine* L1 = tp.cl + (gw + 1) / 3;
+            nz |= L1->pl[2 * ((gw + 1) % 3)] << (64 - b);
+            ng |= L1->pl[2 * ((gw + 1) % 3) + 1] << (64 - b);
+        }
+        const i64 cnt = lim - n0 + 1;
+        if (cnt < 64) {
+            const u64 m = (1ull << cnt) - 1;
+            nz &= m;
+            ng &= m;
+        }
+    }
+    // chunk layout: nonzero bytes of words 0..3, then sign bytes of words 0..3
+    u64* ch = chunks + 8 * (t >> 2);
+    ch[t & 3] = nz;
+    ch[4 + (t & 3)] = ng;
+    ck[t] = m0;
+}
+
 mdr_status run_component(mdr_ctx* c, const Seg& sg, i64 y, int geval, int comb2, i64 u, double cx, long long* lo,
                          long long* hi) {
-    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (!c->tab.valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
     HIPT(c->comp_out.ensure(64), "alloc");
     HIPT(hipMemsetAsync(c->comp_out.p, 0, 16, c->st), "clear");
-    hipLaunchKernelGGL(k_component, dim3(64), dim3(256), 0, c->st, c->tp(), sg, (u64)y, geval, comb2, u, cx,
+    hipLaunchKernelGGL(k_component, dim3(64), dim3(256), 0, c->st, c->tab.tp(), sg, (u64)y, geval, comb2, u, cx,
                        c->comp_out.as<u64>());
     HIPT(hipGetLastError(), "launch component");
     u64 r[2];
@@ -1367,38 +1458,200 @@ extern "C" {
 // limited to 2^31 entries per call.
 constexpr long long MAX_CALL_ITEMS = 1LL << 31;
 
-mdr_status mdr_table_read_M(mdr_ctx* c, long long first, long long count, int32_t* out) {
-    if (!c || count < 0 || count > MAX_CALL_ITEMS || (count && !out) || first < 0) return MDR_ERR_INVALID;
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
-    if (first + count - 1 > c->tU) return fail(c, MDR_ERR_INVALID, "read beyond the table");
+}  // extern "C"
+
+namespace {
+
+// Reads of one table into host memory, through a device staging buffer of
+// at most 2^27 entries per dispatch (the caller holds the context's lock).
+constexpr long long READ_STAGE = 1LL << 27;
+
+mdr_status read_M(mdr_ctx* c, const TableParam& tp, i64 first, i64 count, int32_t* out) {
     if (count == 0) return MDR_OK;
     DevBuf tmp;
-    HIPT(tmp.ensure((size_t)count * 4), "alloc");
-    hipLaunchKernelGGL(k_read_M, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->st, c->tp(), first, count,
-                       tmp.as<int32_t>());
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp.p, (size_t)count * 4, hipMemcpyDeviceToHost, c->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    HIPT(tmp.ensure((size_t)std::min(count, READ_STAGE) * 4), "alloc");
+    hipError_t e = hipSuccess;
+    for (i64 k = 0; k < count && e == hipSuccess; k += READ_STAGE) {
+        const i64 m = std::min(READ_STAGE, count - k);
+        hipLaunchKernelGGL(k_read_M, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->st, tp, first + k, m,
+                           tmp.as<int32_t>());
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(out + k, tmp.p, (size_t)m * 4, hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
     tmp.release();
     if (e != hipSuccess) return hip_fail(c, e, "read M");
     return MDR_OK;
 }
 
+mdr_status read_mu(mdr_ctx* c, const TableParam& tp, i64 first, i64 count, int8_t* out) {
+    if (count == 0) return MDR_OK;
+    DevBuf tmp;
+    HIPT(tmp.ensure((size_t)std::min(count, READ_STAGE)), "alloc");
+    hipError_t e = hipSuccess;
+    for (i64 k = 0; k < count && e == hipSuccess; k += READ_STAGE) {
+        const i64 m = std::min(READ_STAGE, count - k);
+        hipLaunchKernelGGL(k_read_mu, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->st, tp, first + k, m,
+                           tmp.as<int8_t>());
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(out + k, tmp.p, (size_t)m, hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
+    tmp.release();
+    if (e != hipSuccess) return hip_fail(c, e, "read mu");
+    return MDR_OK;
+}
+
+}  // namespace
+
+// A table object: its own device buffers, built and read on its context's GPU.
+struct mdr_table {
+    mdr_ctx* c = nullptr;
+    TableStore s;
+    i64 U = 0, mu = 0;   // as requested (the build may hold one more value)
+    TableParam tp() const {
+        TableParam t = s.tp();
+        t.mu_lim = mu;   // mu reads beyond the requested limit give 0
+        return t;
+    }
+};
+
+extern "C" {
+
+mdr_status mdr_table_read_M(mdr_ctx* c, long long first, long long count, int32_t* out) {
+    if (!c || count < 0 || count > MAX_CALL_ITEMS || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->tab.valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (first + count - 1 > c->tab.U) return fail(c, MDR_ERR_INVALID, "read beyond the table");
+    return read_M(c, c->tab.tp(), first, count, out);
+}
+
 mdr_status mdr_table_read_mu(mdr_ctx* c, long long first, long long count, int8_t* out) {
     if (!c || count < 0 || count > MAX_CALL_ITEMS || (count && !out) || first < 0) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
-    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (!c->tab.valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    return read_mu(c, c->tab.tp(), first, count, out);
+}
+
+mdr_status mdr_table_create(mdr_ctx* c, long long U, long long u_dense, long long mu_limit, mdr_table** out) {
+    if (!c || !out || U < 0 || u_dense < 0 || mu_limit < 0 || mu_limit > U) return MDR_ERR_INVALID;
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    const i64 Ub = std::max<i64>(U, 1);
+    i64 ud = std::min<i64>(u_dense, Ub);
+    if (ud < Ub) ud = (ud / LINE_VALUES) * LINE_VALUES;
+    mdr_table* t = new mdr_table();
+    t->c = c;
+    t->U = U;
+    t->mu = mu_limit;
+    const mdr_status s = build_table(c, t->s, Ub, ud, std::max<i64>(mu_limit, 1), nullptr, false, 0);
+    if (s != MDR_OK) {
+        t->s.release();
+        delete t;
+        return s;
+    }
+    *out = t;
+    return MDR_OK;
+}
+
+void mdr_table_free(mdr_table* t) {
+    if (!t) return;
+    {
+        std::lock_guard<std::mutex> lk(t->c->mu);
+        (void)hipSetDevice(t->c->device);
+        (void)hipStreamSynchronize(t->c->st);
+        t->s.release();
+    }
+    delete t;
+}
+
+mdr_status mdr_table_info(const mdr_table* t, long long* U, long long* u_dense, long long* mu_limit) {
+    if (!t) return MDR_ERR_INVALID;
+    if (U) *U = t->U;
+    if (u_dense) *u_dense = std::min(t->s.ud, t->U);
+    if (mu_limit) *mu_limit = t->mu;
+    return MDR_OK;
+}
+
+mdr_status mdr_table_copy_M(mdr_table* t, long long first, long long count, int32_t* out) {
+    if (!t || count < 0 || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    mdr_ctx* c = t->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    if (count && first + count - 1 > t->U) return fail(c, MDR_ERR_INVALID, "read beyond the table");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    return read_M(c, t->tp(), first, count, out);
+}
+
+mdr_status mdr_table_copy_mu(mdr_table* t, long long first, long long count, int8_t* out) {
+    if (!t || count < 0 || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    mdr_ctx* c = t->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    return read_mu(c, t->tp(), first, count, out);
+}
+
+mdr_status mdr_table_gather_M(mdr_table* t, const long long* q, size_t n, int32_t* out) {
+    if (!t || (n && (!q || !out))) return MDR_ERR_INVALID;
+    mdr_ctx* c = t->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    for (size_t i = 0; i < n; ++i)
+        if (q[i] < 0 || q[i] > t->U) return fail(c, MDR_ERR_INVALID, "gather index outside the table");
+    if (n == 0) return MDR_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    const i64 stage = std::min<i64>((i64)n, READ_STAGE);
+    DevBuf dq, dout;
+    hipError_t e = dq.ensure((size_t)stage * 8);
+    if (e == hipSuccess) e = dout.ensure((size_t)stage * 4);
+    for (i64 k = 0; k < (i64)n && e == hipSuccess; k += stage) {
+        const i64 m = std::min<i64>(stage, (i64)n - k);
+        e = hipMemcpyAsync(dq.p, q + k, (size_t)m * 8, hipMemcpyHostToDevice, c->st);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_gather_M, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->st, t->tp(),
+                               dq.as<i64>(), m, dout.as<int32_t>());
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(out + k, dout.p, (size_t)m * 4, hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
+    dq.release();
+    dout.release();
+    if (e != hipSuccess) return hip_fail(c, e, "gather M");
+    return MDR_OK;
+}
+
+mdr_status mdr_table_copy_chunks(mdr_table* t, long long u_dense_ref, long long first_chunk, long long count,
+                                 void* chunks, int32_t* checkpoints) {
+    if (!t || first_chunk < 0 || count < 0 || (count && (!chunks || !checkpoints))) return MDR_ERR_INVALID;
+    mdr_ctx* c = t->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    if (u_dense_ref < t->s.ud || u_dense_ref > t->U)
+        return fail(c, MDR_ERR_INVALID, "u_dense_ref outside [the table's dense tier, U]");
     if (count == 0) return MDR_OK;
-    DevBuf tmp;
-    HIPT(tmp.ensure((size_t)count), "alloc");
-    hipLaunchKernelGGL(k_read_mu, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->st, c->tp(), first, count,
-                       tmp.as<int8_t>());
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp.p, (size_t)count, hipMemcpyDeviceToHost, c->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    tmp.release();
-    if (e != hipSuccess) return hip_fail(c, e, "read mu");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    const i64 stage = std::min<i64>(count, READ_STAGE / 4);   // chunks per dispatch
+    DevBuf dch, dck;
+    hipError_t e = dch.ensure((size_t)stage * 64);
+    if (e == hipSuccess) e = dck.ensure((size_t)stage * 16);
+    for (i64 k = 0; k < count && e == hipSuccess; k += stage) {
+        const i64 m = std::min<i64>(stage, count - k);
+        hipLaunchKernelGGL(k_export_chunks, dim3((unsigned)((4 * m + 255) / 256)), dim3(256), 0, c->st, t->tp(), t->U,
+                           (i64)u_dense_ref, (i64)(first_chunk + k), 4 * m, dch.as<u64>(), dck.as<int32_t>());
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync((char*)chunks + 64 * k, dch.p, (size_t)m * 64, hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(checkpoints + 4 * k, dck.p, (size_t)m * 16, hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
+    dch.release();
+    dck.release();
+    if (e != hipSuccess) return hip_fail(c, e, "copy chunks");
     return MDR_OK;
 }
 
@@ -1406,9 +1659,9 @@ mdr_status mdr_s2_interval(mdr_ctx* c, int piece, long long first, long long las
                            long long* hi) {
     if (!c || !lo || !hi || piece < 1 || piece > 12 || y < 0) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
-    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (!c->tab.valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
     if (first < 1) first = 1;
-    if (last > c->tmu) return fail(c, MDR_ERR_INVALID, "interval beyond the mu table");
+    if (last > c->tab.mu) return fail(c, MDR_ERR_INVALID, "interval beyond the mu table");
     Seg sg{};
     sg.type = SEG_S2;
     sg.piece = piece;
@@ -1424,10 +1677,10 @@ mdr_status mdr_s1_sum(mdr_ctx* c, long long y, long long first, long long last, 
                       long long* hi) {
     if (!c || !lo || !hi || (stride != 1 && stride != 2) || first < 1 || y < 0) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
-    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
-    if (first <= last && y / first > c->tU) return fail(c, MDR_ERR_INVALID, "quotient beyond the table");
+    if (!c->tab.valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (first <= last && y / first > c->tab.U) return fail(c, MDR_ERR_INVALID, "quotient beyond the table");
     // split at the tier boundary like the engine does
-    const i64 split = y / (c->tud + 1);
+    const i64 split = y / (c->tab.ud + 1);
     u64 rlo = 0;
     i64 rhi = 0;
     for (int comp = 1; comp >= 0; --comp) {
@@ -1443,7 +1696,7 @@ mdr_status mdr_s1_sum(mdr_ctx* c, long long y, long long first, long long last, 
             if (stride == 2 && ((a - first) & 1)) ++a;
         }
         if (a > b) continue;
-        if (comp && c->tL == 0) return fail(c, MDR_ERR_INVALID, "quotient beyond the dense table");
+        if (comp && c->tab.L == 0) return fail(c, MDR_ERR_INVALID, "quotient beyond the dense table");
         sg.first = a;
         sg.count = (b - a) / stride + 1;
         long long l, h;
@@ -1459,9 +1712,9 @@ mdr_status mdr_s1_sum(mdr_ctx* c, long long y, long long first, long long last, 
 mdr_status mdr_g_eval(mdr_ctx* c, long long y, int comb2, long long u, double cx, long long* lo, long long* hi) {
     if (!c || !lo || !hi || y < 4 || u < 2 || !(cx > 0)) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
-    if (!c->tab_valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
-    if (u > c->tU) return fail(c, MDR_ERR_INVALID, "u beyond the table");
-    if ((i64)(cx * std::sqrt((double)y)) > c->tmu) return fail(c, MDR_ERR_INVALID, "split point beyond the mu table");
+    if (!c->tab.valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
+    if (u > c->tab.U) return fail(c, MDR_ERR_INVALID, "u beyond the table");
+    if ((i64)(cx * std::sqrt((double)y)) > c->tab.mu) return fail(c, MDR_ERR_INVALID, "split point beyond the mu table");
     return run_component(c, Seg{}, y, 1, comb2 ? 1 : 0, u, cx, lo, hi);
 }
 

@@ -76,18 +76,9 @@ struct QuotArgs {
     unsigned long long* nstamps;
     i64 stamp_cap;
     int stamp_stride;              // log slots whose hash is 0 mod stride
+    u64* wave_span;                // optional: [2 w] first and [2 w + 1] last wall clock of wave w
 };
 
-// Sieve workgroup shapes (segment length, threads): the table they build is
-// the same; the autotuner picks one per decade of X.  Segment lengths are
-// 3 * 2^shift, multiples of 192 (lines), 48 (mu words) and 16.
-struct SieveShape {
-    int seg, threads, shift;
-};
-constexpr SieveShape SIEVE_SHAPES[] = {{49152, 1024, 14}, {24576, 256, 13}, {98304, 1024, 15},
-                                       {49152, 256, 14},  {49152, 512, 14},  {24576, 512, 13}};
-constexpr int N_SIEVE_SHAPES = 6;
-constexpr int SEG_MAX = 98304;
 hipError_t launch_sieve(int shape, unsigned blocks, hipStream_t st, const SieveArgs& a);
 __global__ void k_table_fixup(FixupArgs a);
 __global__ void k_slot_len(PartArgs a);

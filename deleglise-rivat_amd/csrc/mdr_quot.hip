@@ -442,6 +442,7 @@ __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
     i64 acc_hi = 0;
     i64 cur_x = -1, x_end = 0;
     XParam xp{};
+    const u64 t_start = a.wave_span ? (u64)wall_clock64() : 0;
     for (i64 c = a.dynamic ? fetch_next(a.next, lane) : uniform64(wave); c < a.P;
          c = a.dynamic ? fetch_next(a.next, lane) : c + nwaves) {
         const i64 b0 = uniform64(a.bounds[c]), b1 = uniform64(a.bounds[c + 1]);
@@ -509,6 +510,10 @@ __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
         }
     }
     if (cur_x >= 0) flush(a.out128, cur_x, acc_lo, acc_hi, lane);
+    if (a.wave_span && lane == 0) {
+        a.wave_span[2 * wave] = t_start;
+        a.wave_span[2 * wave + 1] = (u64)wall_clock64();
+    }
 }
 
 // ---------------------------------------------------------------------------

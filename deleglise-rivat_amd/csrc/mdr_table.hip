@@ -303,13 +303,14 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
 }
 
 // Adds the exclusive segment offsets to the segment-local values and sets
-// M(0) = 0.
+// M(0) = 0 (a whole table's fix-up; the sieve writes M(n) from n = 1, so
+// M(0) is set here even when the dense tier holds nothing else).
 // Grid-stride: a dispatch holds fewer than 2^32 work-items, the words can
 // be more.
 __global__ void k_table_fixup(FixupArgs a) {
     const i64 i0 = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     const i64 stride = (i64)gridDim.x * blockDim.x;
-    if (i0 == 0 && a.nd > 0) a.Md[0] = 0;
+    if (i0 == 0 && !a.carry) a.Md[0] = 0;
     for (i64 i = i0; i < a.nd; i += stride) {
         const i64 n = i + 1;
         const i64 s = ((n - 1) >> a.seg_shift) / 3;

@@ -72,7 +72,7 @@ void load() {
         if (k >= 4 && e.fx > 0.05 && e.fx < 100 && e.cx > 0.1 && e.cx < 5.0 && ch >= 0) {
             e.chunk = ch;
             e.ud = k >= 5 && ud >= 0 ? ud : 0;
-            e.sieve = k >= 6 && sv >= 0 && sv < 4 ? sv : 0;
+            e.sieve = k >= 6 && sv >= 0 && sv < N_SIEVE_SHAPES ? sv : 0;
             g_entries.push_back(e);
         }
     }

@@ -26,7 +26,8 @@ EXPORTS = ["mdr_default_opts", "mdr_init", "mdr_shutdown", "mdr_last_error", "md
            "mdr_table_read_mu", "mdr_s2_interval", "mdr_s1_sum", "mdr_g_eval",
            "mdr_cost_model_set", "mdr_cost_model_get", "mdr_cost_model_load", "mdr_cost_model_save",
            "mdr_cycle_log", "mdr_mlp_features", "mdr_mlp_predict", "mdr_mlp_grad", "mdr_mlp_train",
-           "mdr_quot_waves"]
+           "mdr_quot_waves", "mdr_table_create", "mdr_table_free", "mdr_table_info", "mdr_table_copy_M",
+           "mdr_table_copy_mu", "mdr_table_gather_M", "mdr_table_copy_chunks"]
 
 MLP_NPARAM = 1281
 MLP_NFEAT = 5
@@ -40,7 +41,7 @@ class Opts(ctypes.Structure):
                 ("record_cycles", ctypes.c_int), ("no_table_reuse", ctypes.c_int), ("tune_u", ctypes.c_longlong),
                 ("tune_ud", ctypes.c_longlong), ("tune_cx", ctypes.c_double),
                 ("use_graph", ctypes.c_int), ("tune_batch", ctypes.c_double), ("sieve_shape", ctypes.c_int),
-                ("window", ctypes.c_longlong)]
+                ("window", ctypes.c_longlong), ("wave_span", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -56,7 +57,9 @@ class Stats(ctypes.Structure):
                 ("partition_used", ctypes.c_int), ("graph_used", ctypes.c_int),
                 ("graph_instantiated", ctypes.c_int), ("graph_ms", ctypes.c_double),
                 ("windows", ctypes.c_longlong), ("predicted_cycles", ctypes.c_double),
-                ("measured_cycles", ctypes.c_double), ("quotient_bytes", ctypes.c_double)]
+                ("measured_cycles", ctypes.c_double), ("quotient_bytes", ctypes.c_double),
+                ("wave_span_ms", ctypes.c_double), ("wave_busy_ms", ctypes.c_double),
+                ("wave_busy_frac", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -114,6 +117,15 @@ def load():
     lib.mdr_g_eval.argtypes = [vp, ll, i32, ll, dbl, P(ll), P(ll)]
     lib.mdr_g_eval.restype = i32
     lib.mdr_quot_waves.argtypes, lib.mdr_quot_waves.restype = [vp], ll
+    lib.mdr_table_create.argtypes, lib.mdr_table_create.restype = [vp, ll, ll, ll, P(vp)], i32
+    lib.mdr_table_free.argtypes, lib.mdr_table_free.restype = [vp], None
+    lib.mdr_table_info.argtypes, lib.mdr_table_info.restype = [vp, P(ll), P(ll), P(ll)], i32
+    lib.mdr_table_copy_M.argtypes, lib.mdr_table_copy_M.restype = [vp, ll, ll, P(ctypes.c_int32)], i32
+    lib.mdr_table_copy_mu.argtypes, lib.mdr_table_copy_mu.restype = [vp, ll, ll, P(ctypes.c_int8)], i32
+    lib.mdr_table_gather_M.argtypes = [vp, P(ll), ctypes.c_size_t, P(ctypes.c_int32)]
+    lib.mdr_table_gather_M.restype = i32
+    lib.mdr_table_copy_chunks.argtypes = [vp, ll, ll, ll, vp, P(ctypes.c_int32)]
+    lib.mdr_table_copy_chunks.restype = i32
     fp = P(ctypes.c_float)
     sz = ctypes.c_size_t
     lib.mdr_cost_model_set.argtypes, lib.mdr_cost_model_set.restype = [vp, fp, i32], i32
@@ -137,7 +149,7 @@ def _i128(lo, hi):
 def make_opts(strategy=STRATEGY_AUTO, cx=0.0, u_cap=0, fx=0.0, u_cap_dense=0, fx_set=0,
               u_cap_dense_set=0, partition=PARTITION_STATIC, params=PARAMS_REFERENCE,
               sched=SCHED_DYNAMIC, chunk=0, record_cycles=0, no_table_reuse=0, tune_u=0, tune_ud=0,
-              tune_cx=0.0, use_graph=0, tune_batch=0.0, sieve_shape=0, window=0):
+              tune_cx=0.0, use_graph=0, tune_batch=0.0, sieve_shape=0, window=0, wave_span=0):
     o = Opts()
     load().mdr_default_opts(ctypes.byref(o))
     o.strategy, o.cx, o.u_cap, o.fx, o.u_cap_dense = strategy, cx, u_cap, fx, u_cap_dense
@@ -145,6 +157,7 @@ def make_opts(strategy=STRATEGY_AUTO, cx=0.0, u_cap=0, fx=0.0, u_cap_dense=0, fx
     o.sched, o.chunk, o.record_cycles, o.no_table_reuse = sched, chunk, record_cycles, no_table_reuse
     o.tune_u, o.tune_ud, o.tune_cx = tune_u, tune_ud, tune_cx
     o.use_graph, o.tune_batch, o.sieve_shape, o.window = use_graph, tune_batch, sieve_shape, window
+    o.wave_span = wave_span
     return o
 
 
@@ -294,6 +307,69 @@ class Engine:
         self._check(self.lib.mdr_g_eval(self.ctx, y, 1 if comb2 else 0, u, cx, ctypes.byref(lo), ctypes.byref(hi)),
                     "g_eval")
         return _i128(lo, hi)
+
+
+class Table:
+    """A device-resident table object (mdr_table_create): M(n) for n <= U,
+    mu(n) for n <= mu_limit, built on the engine's GPU."""
+
+    def __init__(self, engine, U, u_dense, mu_limit):
+        self.engine = engine
+        self.h = ctypes.c_void_p()
+        engine._check(engine.lib.mdr_table_create(engine.ctx, U, u_dense, mu_limit, ctypes.byref(self.h)),
+                      "mdr_table_create")
+
+    def close(self):
+        if self.h:
+            self.engine.lib.mdr_table_free(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        U, ud, mu = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        self.engine._check(self.engine.lib.mdr_table_info(self.h, ctypes.byref(U), ctypes.byref(ud), ctypes.byref(mu)),
+                           "mdr_table_info")
+        return U.value, ud.value, mu.value
+
+    def M(self, first, count):
+        import numpy as np
+        out = np.empty(count, dtype=np.int32)
+        self.engine._check(self.engine.lib.mdr_table_copy_M(self.h, first, count,
+                                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                           "mdr_table_copy_M")
+        return out
+
+    def mu(self, first, count):
+        import numpy as np
+        out = np.empty(count, dtype=np.int8)
+        self.engine._check(self.engine.lib.mdr_table_copy_mu(self.h, first, count,
+                                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))),
+                           "mdr_table_copy_mu")
+        return out
+
+    def gather_M(self, qs):
+        import numpy as np
+        q = np.ascontiguousarray(qs, dtype=np.int64)
+        out = np.empty(len(q), dtype=np.int32)
+        self.engine._check(self.engine.lib.mdr_table_gather_M(
+            self.h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), len(q),
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "mdr_table_gather_M")
+        return out
+
+    def chunks(self, u_dense_ref, first_chunk, count):
+        """(uint8[count, 64] chunks {nonzero[32], sign[32]}, int32[count * 4] checkpoints)."""
+        import numpy as np
+        ch = np.empty((count, 64), dtype=np.uint8)
+        ck = np.empty(count * 4, dtype=np.int32)
+        self.engine._check(self.engine.lib.mdr_table_copy_chunks(
+            self.h, u_dense_ref, first_chunk, count, ch.ctypes.data_as(ctypes.c_void_p),
+            ck.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "mdr_table_copy_chunks")
+        return ch, ck
 
 
 def _fptr(a):

@@ -14,8 +14,17 @@
  *                      with its TimingStats out-pointer (mertens_cuda.cuh:410-416, :441)
  *   mdr_mertens_batch  the same, for many X sharing one table (no reference
  *                      counterpart beyond repeated compute_mertens calls)
- *   mdr_table_*        SieveTable / CompressedMertensTable get_M and get_mu
- *                      (mertens_dr.hpp:404-425, :707-716)
+ *   mdr_table_build/read_*  the context's cached table, read as SieveTable /
+ *                      CompressedMertensTable get_M and get_mu (mertens_dr.hpp:404-425,
+ *                      :707-716)
+ *   mdr_table_create, _free, _info, _copy_M, _copy_mu, _gather_M, _copy_chunks
+ *                      table objects: the
+ *                      constructors SieveTable(limit, threads, max_mu, ...) and
+ *                      CompressedMertensTable(total_limit, dense_limit, threads,
+ *                      max_mu) (mertens_dr.hpp:386-402, :689-702) with their reads
+ *                      get_M / get_mu / data / dense_data (:404-425, :707-716) and
+ *                      the compressed tier's BitChunk + checkpoint arrays
+ *                      (:670-687, :741-752)
  *   mdr_s2_interval    run_s2_fast<Piece> (mertens_dr.hpp:484-617)
  *   mdr_s1_sum         sum_dense_quotients<Stride> (mertens_dr.hpp:88-128) and the
  *                      compressed-tier loops (:1499-1504)
@@ -35,6 +44,7 @@ extern "C" {
 #endif
 
 typedef struct mdr_ctx mdr_ctx;
+typedef struct mdr_table mdr_table;
 
 typedef enum {
     MDR_OK = 0,
@@ -100,6 +110,9 @@ typedef struct {
                                  is the same, the table's memory is the resident part plus
                                  one window, and the cutoff may exceed the one-table limit
                                  (up to tune_u <= 1e12); such calls run eagerly (no graph) */
+    int wave_span;            /* 1: every wave of the quotient kernel records its first and
+                                 last clock (constant-rate wall clock); stats then report the
+                                 waves' busy fraction and the idle tail (0 = off) */
 } mdr_opts;
 
 typedef struct {
@@ -130,6 +143,12 @@ typedef struct {
      * (quotient_ms x shader clock x resident waves) and the algorithmic bytes
      * (2 B per dense S1 lookup, 20 B per compressed one, 3 B per S2 position) */
     double predicted_cycles, measured_cycles, quotient_bytes;
+    /* opts.wave_span = 1 (last quotient launch of the call): span = last wave
+     * end - first wave start, busy = mean over waves of (end - start); their
+     * ratio is the fraction of the span the average wave worked, so
+     * 1 / wave_busy_frac bounds what any partition of the same work can gain
+     * in the quotient kernel */
+    double wave_span_ms, wave_busy_ms, wave_busy_frac;
 } mdr_stats;
 
 /* One cycle-log record: a run of S1 (n) or S2 (j) positions of one G(y)
@@ -182,6 +201,35 @@ mdr_status mdr_plan(mdr_ctx* ctx, long long X, const mdr_opts* opts, int* stream
 mdr_status mdr_table_build(mdr_ctx* ctx, long long U, long long u_dense, long long mu_limit);
 mdr_status mdr_table_read_M(mdr_ctx* ctx, long long first, long long count, int32_t* out);
 mdr_status mdr_table_read_mu(mdr_ctx* ctx, long long first, long long count, int8_t* out);
+/* Table objects: device-resident tables built by the engine's sieve on the
+ * context's GPU, any number per context besides its cached table, each
+ * living until mdr_table_free (which must come before mdr_shutdown of its
+ * context).  A table holds M(n) for 0 <= n <= U and mu(n) for n <= mu_limit
+ * (mu reads beyond it give 0, as SieveTable::get_mu does); u_dense is the
+ * table's int16 tier (rounded down to a multiple of 192 below U; the values
+ * read are the same for any u_dense).  Calls on a table are serialised with
+ * the other calls on its context.
+ *   create       U >= 0; 0 <= mu_limit <= U (0 = none); *out receives the table
+ *   info         the sizes actually built
+ *   copy_M, _mu  M(first .. first+count-1) as int32, mu(...) as int8, into host
+ *                memory; first >= 0, first + count - 1 <= U for M
+ *   gather_M     out[i] = M(q[i]) for host arrays q, out (0 <= q[i] <= U)
+ *   copy_chunks  the compressed tier of CompressedMertensTable with dense
+ *                limit u_dense_ref (u_dense_ref <= U): chunks
+ *                first_chunk .. first_chunk+count-1 of 256 values from
+ *                u_dense_ref + 1 on, each 64 bytes {nonzero[32], sign[32]}
+ *                (bit i of the chunk: mu != 0, mu < 0 of value i), and four
+ *                int32 checkpoints per chunk (M just before each 64-value
+ *                word); values beyond U read as mu = 0 */
+mdr_status mdr_table_create(mdr_ctx* ctx, long long U, long long u_dense, long long mu_limit, mdr_table** out);
+void mdr_table_free(mdr_table* table);
+mdr_status mdr_table_info(const mdr_table* table, long long* U, long long* u_dense, long long* mu_limit);
+mdr_status mdr_table_copy_M(mdr_table* table, long long first, long long count, int32_t* out);
+mdr_status mdr_table_copy_mu(mdr_table* table, long long first, long long count, int8_t* out);
+mdr_status mdr_table_gather_M(mdr_table* table, const long long* q, size_t n, int32_t* out);
+mdr_status mdr_table_copy_chunks(mdr_table* table, long long u_dense_ref, long long first_chunk, long long count,
+                                 void* chunks, int32_t* checkpoints);
+
 /* sum over j in [first, last], gcd(j,6)=1 of mu(j)*piece(floor(y/j)); piece 1..8 are
  * Piece1..8, 9..12 SinglePiece1..4.  128-bit result as (lo, hi). */
 mdr_status mdr_s2_interval(mdr_ctx* ctx, int piece, long long first, long long last, long long y,
