@@ -20,7 +20,8 @@ public:
     static int64 compute_mertens(int64 x, int threads = 0) {
         (void)threads;
         if (x < 1) return 0;
-        mdr_ctx* c = mertens_dr::hip_detail::context().get();
+        const mertens_dr::hip_detail::Lease lease;
+        mdr_ctx* c = lease.get();
         long long out = 0;
         mertens_dr::hip_detail::check(mdr_mertens_dirichlet(c, x, &out, nullptr), c, "DirichletEngine::compute_mertens");
         return out;

@@ -33,7 +33,8 @@ static inline int64 compute_mertens_windowed(int64 X, int num_threads = 0, int64
                                              bool verbose = false) {
     (void)num_threads;
     if (X < 1) return 0;
-    mdr_ctx* c = mertens_dr::hip_detail::context().get();
+    const mertens_dr::hip_detail::Lease lease;
+    mdr_ctx* c = lease.get();
     mdr_opts o = mertens_dr::hip_detail::opts_from_env(MDR_STRATEGY_DENSE);
     // the windowed engine keeps the reference's dense-strategy cutoff unless
     // MERTENS_HIP_TUNED=1 asks for the tuned one
