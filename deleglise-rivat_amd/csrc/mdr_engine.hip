@@ -131,6 +131,7 @@ struct mdr_ctx {
     // work
     DevBuf xparam, slot_len, slot_cost, bounds, out128, ctr, stamps, comp_out, lookup, wspan;
     int wall_khz = 0;   // rate of the wall clock the quotient waves stamp
+    void* na_host = nullptr;   // pinned result words of mdr_mertens_noalloc
     std::vector<StampRec> last_stamps;
     // instantiated graphs, one per launch sequence (see launch_graph)
     hipGraphExec_t graph_exec[16] = {};
@@ -1170,6 +1171,226 @@ mdr_status dirichlet_mertens(mdr_ctx* c, i64 X, long long* out, mdr_stats* stats
     return MDR_OK;
 }
 
+// ---- allocation-free entry point ----------------------------------------------------
+// mdr_mertens_noalloc computes M(X) with no device allocation: every device
+// buffer of the call -- the stencil and sieving primes (generated by kernels,
+// no upload), the table, the partition arrays, the counters and the result --
+// is carved out of one file-scope device array of NA_SCRATCH_BYTES.  Each GPU
+// holds its own copy of the array (loaded with the code object, held until the
+// process ends); its address on a device is read once, with that device
+// current, by mdr_init (hipGetSymbolAddress) and kept per device, and only
+// that address is handed to kernels.  Calls on one device share the array:
+// they are serialised by a per-device lock inside this process, and calls
+// from several processes each use their own copy.
+namespace {
+
+constexpr size_t NA_SCRATCH_BYTES = (size_t)256 << 20;
+constexpr int NA_MAX_DEVICES = 64;
+// sieving primes up to this root (one workgroup sieves them in LDS), so the
+// table of a call stays below NA_MAX_ROOT^2 values
+constexpr i64 NA_MAX_ROOT = 65535;
+
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+void* g_na_ptr[NA_MAX_DEVICES] = {};
+std::mutex g_na_mu[NA_MAX_DEVICES];
+std::mutex g_na_init_mu;
+
+// Reads this device's address of the scratch array (the device is current).
+hipError_t noalloc_setup(int device) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_na_init_mu);
+    if (g_na_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_na_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+// The periodic stencil (weights of 2, 3, 5, 7, 11 and the square flags of 4,
+// 9, 25) for n = x + 1, as ensure_primes builds it on the host.
+__global__ void k_na_stencil(uint8_t* st, i64 n) {
+    const i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const unsigned v = (unsigned)(x % STENCIL_PERIOD) + 1;
+    unsigned a = 0;
+    if (v % 2 == 0) a += 1;    // bit_width(p - 1) | 1 for p = 2, 3, 5, 7, 11
+    if (v % 3 == 0) a += 3;
+    if (v % 5 == 0) a += 3;
+    if (v % 7 == 0) a += 3;
+    if (v % 11 == 0) a += 5;
+    if (v % 4 == 0 || v % 9 == 0 || v % 25 == 0) a |= 0x80u;
+    st[x] = (uint8_t)a;
+}
+
+// The sieving primes up to root (< 65536) in the order ensure_primes uploads
+// them: primes >= 13 with 1/p and their byte weights, primes >= 7 for the
+// square flags.  One workgroup: an LDS sieve of root + 1 flags, then an
+// ordered compaction, 1024 numbers per step.
+__global__ __launch_bounds__(1024) void k_na_primes(int root, uint32_t* wp, double* wr, uint8_t* ww, uint32_t* sq) {
+    __shared__ uint8_t comp[NA_MAX_ROOT + 1];
+    __shared__ int cnt_w[1024 / 64], cnt_s[1024 / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i <= root; i += 1024) comp[i] = i < 2 ? 1 : 0;
+    __syncthreads();
+    for (int p = 2; p * p <= root; ++p) {
+        if (comp[p]) continue;   // read by every thread after the previous barrier
+        for (int m = p * p + tid * p; m <= root; m += 1024 * p) comp[m] = 1;
+        __syncthreads();
+    }
+    int base_w = 0, base_s = 0;
+    for (int b = 0; b <= root; b += 1024) {
+        const int n = b + tid;
+        const bool pr = n <= root && !comp[n];
+        const bool in_w = pr && n >= 13, in_s = pr && n >= 7;
+        const u64 mw = __ballot(in_w), ms = __ballot(in_s);
+        const u64 below = (lane ? ~0ull >> (64 - lane) : 0ull);
+        if (lane == 0) {
+            cnt_w[wave] = __popcll(mw);
+            cnt_s[wave] = __popcll(ms);
+        }
+        __syncthreads();
+        int ow = base_w, os = base_s, tw = 0, ts = 0;
+        for (int w = 0; w < 1024 / 64; ++w) {
+            if (w < wave) { ow += cnt_w[w]; os += cnt_s[w]; }
+            tw += cnt_w[w];
+            ts += cnt_s[w];
+        }
+        if (in_w) {
+            const int k = ow + __popcll(mw & below);
+            wp[k] = (uint32_t)n;
+            wr[k] = 1.0 / (double)n;
+            ww[k] = (uint8_t)((32u - (uint32_t)__clz((uint32_t)n - 1u)) | 1u);
+        }
+        if (in_s) sq[os + __popcll(ms & below)] = (uint32_t)n;
+        base_w += tw;
+        base_s += ts;
+        __syncthreads();
+    }
+}
+
+__global__ void k_na_xparam(XParam xp, XParam* dst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) dst[0] = xp;
+}
+
+// M(q) of the table into a 128-bit result (an X read from the table).
+__global__ void k_na_read(TableParam tp, i64 q, u64* out128) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const i64 v = get_M(tp, q);
+        out128[0] = (u64)v;
+        out128[1] = (u64)(v >> 63);
+    }
+}
+
+// Byte offsets of one call's arrays in the scratch array.
+struct NaLayout {
+    size_t stencil = 0, wp = 0, wr = 0, ww = 0, sq = 0, Md = 0, cl = 0, mu6 = 0, segsum = 0, scan = 0;
+    size_t xparam = 0, len = 0, bounds = 0, ctr = 0, out = 0, end = 0;
+};
+
+size_t na_take(size_t& at, size_t bytes) {
+    const size_t off = at;
+    at += (bytes + 255) & ~(size_t)255;
+    return off;
+}
+
+// Counts of the sieving primes up to root, as the kernel lays them out.
+void na_prime_counts(i64 root, int& n_w, int& n_small, int& n_sq, int& n_sq_small) {
+    uint64_t comp[(NA_MAX_ROOT + 64) / 64] = {};
+    n_w = n_small = n_sq = n_sq_small = 0;
+    for (i64 i = 2; i <= root; ++i) {
+        if (comp[i >> 6] >> (i & 63) & 1) continue;
+        for (i64 m = i * i; m <= root; m += i) comp[m >> 6] |= 1ull << (m & 63);
+        if (i >= 13) { ++n_w; if (i < 256) ++n_small; }
+        if (i >= 7) { ++n_sq; if (i < 256) ++n_sq_small; }
+    }
+}
+
+struct NaPlan {
+    int kind = 0;       // 0 constant, 1 table lookup, 2 Deleglise-Rivat sum
+    i64 value = 0;
+    i64 U = 0, ud = 0, mu_lim = 0, u = 0, N = 0, T = 0, slots = 0, P = 0, nseg = 0, musegs = 0, L = 0;
+    double cx = 0;
+    int shape = 0;
+    NaLayout lay;
+};
+
+// Lays out a plan's arrays; false when they do not fit the scratch array.
+bool na_layout(NaPlan& q, i64 nwaves) {
+    const SieveShape& sh = SIEVE_SHAPES[q.shape];
+    q.nseg = (q.U + sh.seg - 1) / sh.seg;
+    q.musegs = (q.mu_lim + sh.seg - 1) / sh.seg;
+    q.L = q.U > q.ud ? (q.U - q.ud + LINE_VALUES - 1) / LINE_VALUES : 0;
+    const i64 root = isqrt_host(q.U) + 1;
+    if (root > NA_MAX_ROOT) return false;
+    NaLayout& l = q.lay;
+    size_t at = 0;
+    l.stencil = na_take(at, (size_t)STENCIL_PERIOD + SEG_MAX);
+    l.wp = na_take(at, (size_t)root * 4 + 4);
+    l.wr = na_take(at, (size_t)root * 8 + 8);
+    l.ww = na_take(at, (size_t)root + 4);
+    l.sq = na_take(at, (size_t)root * 4 + 4);
+    l.Md = na_take(at, (size_t)(q.ud + 1) * 2);
+    l.cl = na_take(at, (size_t)q.L * sizeof(CLine));
+    l.mu6 = na_take(at, (size_t)q.musegs * (sh.seg / 48) * 4);
+    l.segsum = na_take(at, (size_t)q.nseg * 8);
+    q.P = q.kind == 2 ? std::max<i64>(1, nwaves * 64) : 0;
+    l.scan = na_take(at, (size_t)std::max(scan_tmp_len(q.nseg), scan_tmp_len(q.slots)) * 8);
+    l.xparam = na_take(at, sizeof(XParam));
+    l.len = na_take(at, (size_t)std::max<i64>(q.slots, 1) * 8);
+    l.bounds = na_take(at, (size_t)(q.P + 1) * 8);
+    l.ctr = na_take(at, 64);
+    l.out = na_take(at, 16);
+    l.end = at;
+    return at <= NA_SCRATCH_BYTES;
+}
+
+// The plan of an allocation-free call: the tuned split, with the cutoff
+// lowered (M(X) does not depend on it) until the call's arrays fit.
+mdr_status na_plan(mdr_ctx* c, i64 X, NaPlan& q, i64 nwaves) {
+    mdr_opts o;
+    mdr_default_opts(&o);
+    o.params = MDR_PARAMS_TUNED;
+    const XPlan p = plan_x(X, o);
+    q = NaPlan{};
+    q.kind = p.kind;
+    q.value = p.value;
+    if (p.kind == 0) return MDR_OK;
+    q.shape = tune_sieve_shape(X);
+    if (p.kind == 1) {
+        q.U = q.ud = q.mu_lim = std::max<i64>(p.X, 1);
+        if (!na_layout(q, nwaves))
+            return fail(c, MDR_ERR_RANGE, "X beyond what the allocation-free call's scratch array holds");
+        return MDR_OK;
+    }
+    q.cx = p.cx;
+    const i64 A2 = (i64)(p.cx * std::sqrt((double)X)) + 1000;
+    for (i64 u = p.u;; u = u / 5 * 4) {
+        u = std::max<i64>(u / LINE_VALUES * LINE_VALUES, LINE_VALUES);
+        q.u = u;
+        q.N = X / u;
+        q.mu_lim = std::max(A2, q.N + 1000);
+        q.U = std::max(u, q.mu_lim);
+        q.ud = std::min<i64>(std::min(p.ud, u), q.U);
+        if (q.ud < q.U) q.ud = q.ud / LINE_VALUES * LINE_VALUES;
+        q.T = count_coprime6(q.N);
+        q.slots = 2 * q.T;
+        if (X <= q.U) {   // the lowered cutoff reaches X: a table read
+            q.kind = 1;
+            q.U = q.ud = q.mu_lim = X;
+            q.slots = 0;
+        }
+        if (na_layout(q, nwaves)) return MDR_OK;
+        if (u <= A2) break;   // the split's own limit: lowering u further only adds slots
+    }
+    return fail(c, MDR_ERR_RANGE,
+                "X beyond what the allocation-free call's scratch array holds (" +
+                    std::to_string(NA_SCRATCH_BYTES >> 20) + " MiB; about X <= 8e15)");
+}
+
+}  // namespace
+
 // ---- C ABI --------------------------------------------------------------------------
 extern "C" {
 
@@ -1203,6 +1424,13 @@ mdr_status mdr_init(mdr_ctx** out, int device) {
         return MDR_ERR_HIP;
     }
     c->quot_blocks_per_cu = std::max(1, occ);
+    // the allocation-free entry point: this device's address of its scratch
+    // array, and a pinned word pair for its result
+    if (noalloc_setup(device) != hipSuccess || hipHostMalloc(&c->na_host, 64, hipHostMallocDefault) != hipSuccess) {
+        c->na_host = nullptr;
+        mdr_shutdown(c);
+        return MDR_ERR_HIP;
+    }
     size_t mem_free = 0, mem_total = 0;
     if (hipMemGetInfo(&mem_free, &mem_total) == hipSuccess && mem_total > 0) tune_set_device_memory(mem_total);
     // the trained cost model: MDR_COST_MODEL, else cost_model.bin beside the library
@@ -1235,6 +1463,7 @@ void mdr_shutdown(mdr_ctx* c) {
     for (DevBuf* b : bufs) b->release();
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (c->na_host) (void)hipHostFree(c->na_host);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -1287,6 +1516,178 @@ mdr_status mdr_mertens_batch(mdr_ctx* c, const long long* X, size_t n, const mdr
 
 mdr_status mdr_mertens(mdr_ctx* c, long long X, const mdr_opts* opts, long long* out, mdr_stats* stats) {
     return mdr_mertens_batch(c, &X, 1, opts, out, stats);
+}
+
+mdr_status mdr_mertens_noalloc(mdr_ctx* c, long long X, long long* out, mdr_stats* stats) {
+    if (!c || !out) return MDR_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    if (stats) *stats = mdr_stats{};
+    const auto t0 = std::chrono::steady_clock::now();
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
+    if (c->device >= NA_MAX_DEVICES || !g_na_ptr[c->device] || !c->na_host)
+        return fail(c, MDR_ERR_HIP, "allocation-free scratch array not set up on this device");
+    std::lock_guard<std::mutex> dlk(g_na_mu[c->device]);
+    const i64 nwaves = (i64)c->ncu * c->quot_blocks_per_cu * 4;
+    NaPlan q;
+    mdr_status s = na_plan(c, X, q, nwaves);
+    if (s != MDR_OK) return s;
+    if (q.kind == 0) {
+        *out = q.value;
+        return MDR_OK;
+    }
+    unsigned char* sc = static_cast<unsigned char*>(g_na_ptr[c->device]);
+    const NaLayout& l = q.lay;
+    const SieveShape& sh = SIEVE_SHAPES[q.shape];
+    // stencil and sieving primes, generated on the device
+    const i64 nst = (i64)STENCIL_PERIOD + SEG_MAX;
+    const i64 root = isqrt_host(q.U) + 1;
+    int n_w = 0, n_small = 0, n_sq = 0, n_sq_small = 0;
+    na_prime_counts(root, n_w, n_small, n_sq, n_sq_small);
+    HIPT(hipEventRecord(c->ev[0], c->st), "event");
+    hipLaunchKernelGGL(k_na_stencil, dim3((unsigned)((nst + 255) / 256)), dim3(256), 0, c->st, sc + l.stencil, nst);
+    hipLaunchKernelGGL(k_na_primes, dim3(1), dim3(1024), 0, c->st, (int)root, (uint32_t*)(sc + l.wp),
+                       (double*)(sc + l.wr), sc + l.ww, (uint32_t*)(sc + l.sq));
+    HIPT(hipGetLastError(), "launch prime setup");
+    // the table
+    SieveArgs a;
+    a.U = q.U;
+    a.ud = q.ud;
+    a.L = q.L;
+    a.seg_first = 0;
+    a.mu_segs = q.musegs;
+    a.stencil = sc + l.stencil;
+    a.wprimes = (const uint32_t*)(sc + l.wp);
+    a.wrcp = (const double*)(sc + l.wr);
+    a.wweights = sc + l.ww;
+    a.n_w = n_w;
+    a.n_small = n_small;
+    a.sqprimes = (const uint32_t*)(sc + l.sq);
+    a.n_sq = n_sq;
+    a.n_sq_small = n_sq_small;
+    a.Md = (int16_t*)(sc + l.Md);
+    a.cl = (CLine*)(sc + l.cl);
+    a.mu6 = (uint32_t*)(sc + l.mu6);
+    a.segsum = (i64*)(sc + l.segsum);
+    a.seg0 = 0;
+    HIPT(launch_sieve(q.shape, (unsigned)q.nseg, c->st, a), "launch sieve");
+    i64* scan = (i64*)(sc + l.scan);
+    const i64 scan_len = (i64)((l.xparam - l.scan) / 8);
+    HIPT(scan_exclusive_i64(a.segsum, a.segsum, q.nseg, scan, scan_len, nullptr, c->st), "scan segments");
+    FixupArgs f;
+    f.ud = q.ud;
+    f.L = q.L;
+    f.nd = q.ud;
+    f.seg_shift = sh.shift;
+    f.Md = a.Md;
+    f.cl = a.cl;
+    f.segoff = a.segsum;
+    f.seg0 = 0;
+    f.carry = nullptr;
+    const i64 nfix = std::max<i64>(1, std::max(q.ud, q.L));
+    hipLaunchKernelGGL(k_table_fixup, dim3((unsigned)std::min<i64>((nfix + 255) / 256, 1 << 20)), dim3(256), 0, c->st,
+                       f);
+    HIPT(hipGetLastError(), "launch fixup");
+    HIPT(hipEventRecord(c->ev[1], c->st), "event");
+    TableParam tp;
+    tp.U = q.U;
+    tp.ud = q.ud;
+    tp.L = q.L;
+    tp.mu_lim = q.mu_lim;
+    tp.Md = a.Md;
+    tp.cl = a.cl;
+    tp.mu6 = a.mu6;
+    tp.pass = 0;
+    tp.qlo = q.ud + 1;
+    tp.qhi = q.U;
+    u64* out128 = (u64*)(sc + l.out);
+    u64* ctr = (u64*)(sc + l.ctr);
+    hipLaunchKernelGGL(k_clear3, dim3(1), dim3(64), 0, c->st, ctr, (i64)8, out128, (i64)2, (u64*)nullptr, (i64)0);
+    HIPT(hipGetLastError(), "launch clear");
+    if (q.kind == 1) {
+        hipLaunchKernelGGL(k_na_read, dim3(1), dim3(64), 0, c->st, tp, (i64)X, out128);
+        HIPT(hipGetLastError(), "launch read");
+    } else {
+        XParam xp{};
+        xp.X = X;
+        xp.u = q.u;
+        xp.N = q.N;
+        xp.T = q.T;
+        xp.slot_base = 0;
+        xp.cx = q.cx;
+        xp.n6 = q.N / 6;
+        xp.n3 = q.N / 3;
+        xp.n2 = q.N / 2;
+        XParam* dxp = (XParam*)(sc + l.xparam);
+        hipLaunchKernelGGL(k_na_xparam, dim3(1), dim3(64), 0, c->st, xp, dxp);
+        PartArgs pa{};
+        pa.xp = dxp;
+        pa.nx = 1;
+        pa.total_slots = q.slots;
+        pa.tp = tp;
+        pa.mlp = nullptr;
+        pa.len = (i64*)(sc + l.len);
+        pa.cost = nullptr;
+        pa.type_pos = (unsigned long long*)(ctr + 1);
+        pa.totals = (const i64*)(ctr + 4);
+        pa.P = q.P;
+        pa.bounds = (i64*)(sc + l.bounds);
+        pa.min_chunk = MIN_CHUNK_POSITIONS;
+        pa.by_slot = 0;
+        hipLaunchKernelGGL(k_slot_len, dim3((unsigned)((q.slots + 255) / 256)), dim3(256), 0, c->st, pa);
+        HIPT(hipGetLastError(), "launch slot lengths");
+        HIPT(scan_exclusive_i64(pa.len, pa.len, q.slots, scan, scan_len, (i64*)(ctr + 4), c->st), "scan positions");
+        hipLaunchKernelGGL(k_cuts, dim3((unsigned)((q.P + 1 + 255) / 256)), dim3(256), 0, c->st, pa);
+        HIPT(hipGetLastError(), "launch cuts");
+        QuotArgs qa{};
+        qa.xp = dxp;
+        qa.nx = 1;
+        qa.tp = tp;
+        qa.off = pa.len;
+        qa.nslots = q.slots;
+        qa.bounds = pa.bounds;
+        qa.P = q.P;
+        qa.next = (unsigned long long*)ctr;
+        qa.out128 = out128;
+        qa.dynamic = 1;
+        qa.stamps = nullptr;
+        qa.nstamps = (unsigned long long*)(ctr + 6);
+        qa.stamp_cap = 0;
+        qa.stamp_stride = 1;
+        qa.wave_span = nullptr;
+        HIPT(hipEventRecord(c->ev[2], c->st), "event");
+        hipLaunchKernelGGL(k_quot, dim3((unsigned)(c->ncu * c->quot_blocks_per_cu)), dim3(256), 0, c->st, qa);
+        HIPT(hipGetLastError(), "launch quotient kernel");
+        HIPT(hipEventRecord(c->ev[3], c->st), "event");
+    }
+    HIPT(hipMemcpyAsync(c->na_host, out128, 16, hipMemcpyDeviceToHost, c->st), "download");
+    HIPT(hipStreamSynchronize(c->st), "allocation-free call");
+    const u64* r = static_cast<const u64*>(c->na_host);
+    const i64 lo = (i64)r[0], hi = (i64)r[1];
+    if (hi != (lo >> 63)) return fail(c, MDR_ERR_RANGE, "result does not fit in 64 bits");
+    *out = lo;
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+        stats->table_ms = ms;
+        if (q.kind == 2) {
+            (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+            stats->quotient_ms = ms;
+        }
+        stats->U = q.U;
+        stats->u_dense = q.ud;
+        stats->mu_limit = q.mu_lim;
+        stats->W = q.L;
+        stats->u = q.u;
+        stats->N = q.N;
+        stats->slots = q.slots;
+        stats->chunks = q.P;
+        stats->cx = q.cx;
+        stats->partition_used = MDR_PARTITION_STATIC;
+        stats->kernel_time = (stats->table_ms + stats->quotient_ms) / 1000.0;
+        stats->total_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return MDR_OK;
 }
 
 mdr_status mdr_mertens_dirichlet(mdr_ctx* c, long long X, long long* out, mdr_stats* stats) {

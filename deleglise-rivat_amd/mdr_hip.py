@@ -27,7 +27,7 @@ EXPORTS = ["mdr_default_opts", "mdr_init", "mdr_shutdown", "mdr_last_error", "md
            "mdr_cost_model_set", "mdr_cost_model_get", "mdr_cost_model_load", "mdr_cost_model_save",
            "mdr_cycle_log", "mdr_mlp_features", "mdr_mlp_predict", "mdr_mlp_grad", "mdr_mlp_train",
            "mdr_quot_waves", "mdr_table_create", "mdr_table_free", "mdr_table_info", "mdr_table_copy_M",
-           "mdr_table_copy_mu", "mdr_table_gather_M", "mdr_table_copy_chunks"]
+           "mdr_table_copy_mu", "mdr_table_gather_M", "mdr_table_copy_chunks", "mdr_mertens_noalloc"]
 
 MLP_NPARAM = 1281
 MLP_NFEAT = 5
@@ -100,6 +100,8 @@ def load():
     lib.mdr_mertens.restype = i32
     lib.mdr_mertens_batch.argtypes = [vp, P(ll), ctypes.c_size_t, P(Opts), P(ll), P(Stats)]
     lib.mdr_mertens_batch.restype = i32
+    lib.mdr_mertens_noalloc.argtypes = [vp, ll, P(ll), P(Stats)]
+    lib.mdr_mertens_noalloc.restype = i32
     lib.mdr_mertens_dirichlet.argtypes = [vp, ll, P(ll), P(Stats)]
     lib.mdr_mertens_dirichlet.restype = i32
     lib.mdr_plan.argtypes = [vp, ll, P(Opts), P(i32), P(ll), P(ll), P(ll), P(ll), P(dbl)]
@@ -241,6 +243,15 @@ class Engine:
         stats = Stats()
         self._check(self.lib.mdr_mertens(self.ctx, X, ctypes.byref(o), ctypes.byref(out), ctypes.byref(stats)),
                     f"mdr_mertens({X})")
+        self.last_stats = stats
+        return out.value
+
+    def mertens_noalloc(self, X):
+        """M(X) through the allocation-free entry point (mdr_mertens_noalloc)."""
+        out = ctypes.c_longlong()
+        stats = Stats()
+        self._check(self.lib.mdr_mertens_noalloc(self.ctx, X, ctypes.byref(out), ctypes.byref(stats)),
+                    f"mdr_mertens_noalloc({X})")
         self.last_stats = stats
         return out.value
 

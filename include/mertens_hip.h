@@ -174,6 +174,22 @@ mdr_status mdr_mertens(mdr_ctx* ctx, long long X, const mdr_opts* opts, long lon
 /* M(X[i]) for n values sharing one table sized to the largest cutoff. */
 mdr_status mdr_mertens_batch(mdr_ctx* ctx, const long long* X, size_t n, const mdr_opts* opts,
                              long long* out, mdr_stats* stats);
+/* M(X) with no device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory during the call (code running
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every device buffer of the call lives in one 256 MiB array
+ * that each GPU holds for the life of the process (set up by mdr_init), so
+ * the call performs no hipMalloc, hipFree or other allocation; inputs and
+ * outputs are the caller's.  The plan is the tuned split with the cutoff
+ * lowered until the call fits the array (M(X) does not depend on it); X
+ * beyond what fits (about X > 8e15) fails with MDR_ERR_RANGE.  Static
+ * partition, dynamic chunk queue, table rebuilt every call.  All calls on
+ * one GPU share its array: calls through contexts of one process are
+ * serialised per device, and calls that overlap on one GPU by other means
+ * would race.  stats (optional): table_ms, quotient_ms, U, u_dense,
+ * mu_limit, W, u, N, slots, chunks, cx. */
+mdr_status mdr_mertens_noalloc(mdr_ctx* ctx, long long X, long long* out, mdr_stats* stats);
+
 /* Cross-check engine: dirichlet::DirichletEngine::compute_mertens (the
  * reference's optional dirichlet_engine.hpp, included by bench_dr.cpp:9-12 and
  * test_dr.cpp:8-11 and cross-validated at test_dr.cpp:71-88; not shipped with
