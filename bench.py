@@ -8,13 +8,20 @@ One step = one exact M(1e17) through the C ABI, table build included (the
 table is rebuilt every step; X is the only input and lives on the host as an
 integer argument, so nothing needs staging).  `value` is evaluations per
 second over the K timed steps.  Beside it:
-  - the static-partition times of the same X: equal quotient positions per
-    chunk, and whole terms per chunk (the reference GPU engine's unit), under
-    the dynamic chunk queue and under a one-chunk-per-wave round robin,
-  - batched evals/s for the 4096-X workload (seed 3, [1e9, 1e14]) against
-    one-at-a-time calls,
-  - the quotient kernel's roofline (algorithmic bytes / kernel time),
-  - the compiled reference CPU engine timed on this host (cpu_baseline).
+  - the partitions of the same X: equal quotient positions per chunk, equal
+    predicted cycles (learned) and whole terms per chunk (the reference GPU
+    engine's unit), under the dynamic chunk queue and under a one-chunk-per-
+    wave round robin, with each launch's wave busy fraction (the idle tail
+    bounds what any partition can gain),
+  - the other configurations: M(1e12) dense and M(1e15) streaming with the
+    reference's parameters (warm median of 3 and cold), the 4096-X batch in
+    [1e9, 1e14] against one-at-a-time calls, the 64-X batch in [1e16, 1e18],
+    each batch under the learned and the static partition,
+  - the quotient kernel's roofline against its binding resource: VALU issue
+    (a lower bound on instructions from the inner loops' instruction counts)
+    and DRAM bytes (counter profile committed under profiles/),
+  - the reference CPU engine, compiled from the reference sources, timed on
+    this host's cores at the same configurations (cpu_baseline).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -22,6 +29,7 @@ import argparse
 import json
 import os
 import re
+import statistics
 import subprocess
 import sys
 import time
@@ -33,13 +41,18 @@ import mdr_hip  # noqa: E402
 
 METRIC = "M(X) wall time at X=1e17 learned vs static S2 balance; batched M(X) evals/sec"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per quotient position: S1 dense-tier lookup 2 B (int16),
-# S1 compressed-tier lookup 20 B (two 8-byte planes + a 4-byte checkpoint),
-# S2 position 3 B (the reference's mu byte stream, A bytes per A/3 positions).
-BYTES_S1_DENSE, BYTES_S1_COMP, BYTES_S2 = 2.0, 20.0, 3.0
-# Reference CPU engine: t(M(1e17)) / t(M(1e16)) measured on 8 threads of the
-# build container (174.93 s / 20.99 s); used to scale the bounded 1e16 sample.
-REF_RATIO_17_16 = 174.93 / 20.99
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every
+# 4 cycles (16 lanes per cycle; the unit SQ_ACTIVE_INST_VALU counts in), at
+# the 2.4 GHz peak engine clock.
+SIMDS, CLOCK_HZ, CYCLES_PER_INST = 1024, 2.4e9, 4
+ISSUE_PEAK_GINST = SIMDS * CLOCK_HZ / CYCLES_PER_INST / 1e9
+# VALU instructions per quotient position in the quotient kernel's inner loops
+# (gfx950 build of csrc/mdr_quot.hip, counted per loop iteration and divided by
+# the positions an iteration takes): dense S1 25 per 2 positions, compressed
+# S1 62 per 2, S2 52..92 per 4 depending on the piece (the cheapest is used,
+# so the issue time below is a lower bound).
+VALU_PER_S1_DENSE, VALU_PER_S1_COMP, VALU_PER_S2_MIN = 12.5, 31.0, 13.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_mertens_1e17.json")
 
 
 def log(*a):
@@ -51,50 +64,169 @@ def load_json(name):
         return json.load(f)
 
 
-def cpu_baseline(threads):
+# ---- CPU baseline ---------------------------------------------------------------
+def host_info():
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["nproc_affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for key, name in (("Model name", "model"), ("Socket(s)", "sockets"), ("Core(s) per socket", "cores_per_socket"),
+                          ("Thread(s) per core", "threads_per_core")):
+            m = re.search(rf"^{re.escape(key)}:\s*(.+)$", out, re.M)
+            if m:
+                info[name] = m.group(1).strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads():
+    # the host share this process may use: OMP_NUM_THREADS when the host sets
+    # it (the GPU box gives each GPU 16 cores), else every CPU of the affinity set
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def ref_single(exe, X, threads, env=None):
+    out = subprocess.run([exe, str(X), str(threads)], capture_output=True, text=True, timeout=900,
+                         env=dict(os.environ, **(env or {}))).stdout
+    return (int(re.search(r"M\(\d+\) = (-?\d+)", out).group(1)),
+            float(re.search(r"Elapsed time: ([0-9.]+)", out).group(1)))
+
+
+def ref_median(exe, exp, threads, repeats, env=None):
+    # the reference's own method: decades 10^1 .. 10^exp in one process, median of `repeats`
+    out = subprocess.run([exe, "--benchmark", str(exp), str(threads), str(repeats)], capture_output=True, text=True,
+                         timeout=900, env=dict(os.environ, **(env or {}))).stdout
+    row = [l for l in out.splitlines() if l.startswith(f"| 10^{exp} ")][0]
+    cells = [c.strip() for c in row.strip("|").split("|")]
+    return int(cells[2]), float(cells[3].split()[0])
+
+
+def cpu_baseline(threads, expect):
+    """The reference's OpenMP engine, compiled from its sources by oracle/Makefile,
+    timed on this host: M(1e17) single run (the headline configuration), M(1e12)
+    dense and M(1e15) streaming as medians of 3 (benchmarks/streaming-2026-10-19.md
+    :15-27, bench_dr.cpp:48-73)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "bench_dr")
-    X = 10**16
-    if os.path.exists(exe):
-        out = subprocess.run([exe, str(X), str(threads)], capture_output=True, text=True, timeout=600).stdout
-        t = float(re.search(r"Elapsed time: ([0-9.]+)", out).group(1))
-        m = int(re.search(r"M\(\d+\) = (-?\d+)", out).group(1))
-        kind = "reference"
-    else:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        m, t = oracle.time_mertens(X, threads)
-        kind = "port"
-    est17 = t * REF_RATIO_17_16
-    return {"value": 1.0 / est17, "unit": "M(X)/s", "cores": threads, "kind": kind,
-            "sample": f"M(1e16) = {m} in {t:.2f} s on {threads} threads, scaled to 1e17 by the reference's "
-                      f"own 1e17/1e16 time ratio {REF_RATIO_17_16:.3f} (est. {est17:.1f} s per M(1e17))",
-            "seconds_1e16": t, "est_seconds_1e17": est17}
+    if not os.path.exists(exe):
+        return None
+    base = {"unit": "M(X)/s", "cores": threads, "kind": "reference", "host": host_info()}
+    ref_single(exe, 10**8, threads)   # warm-up
+    m12, t12 = ref_median(exe, 12, threads, 3, {"MERTENS_USE_CHMT": "0"})
+    m15, t15 = ref_median(exe, 15, threads, 3)
+    m17, t17 = ref_single(exe, 10**17, threads)
+    for X, m in ((10**12, m12), (10**15, m15), (10**17, m17)):
+        if X in expect:
+            assert m == expect[X], (X, m)
+    base.update({
+        "value": 1.0 / t17,
+        "sample": f"the whole M(1e17) = {m17} on {threads} threads: {t17:.2f} s (single run, as the reference "
+                  f"times 1e16 and above); M(1e12) dense and M(1e15) streaming as medians of 3",
+        "seconds": {"1e17": round(t17, 3), "1e15_streaming_median3": round(t15, 4),
+                    "1e12_dense_median3": round(t12, 5)},
+    })
+    return base
 
 
-def roofline(stats):
-    """k_quot over the timed steps: algorithmic bytes per launch / mean launch
-    time (hipEvents recorded around the launch on the engine's stream)."""
+# ---- rooflines --------------------------------------------------------------------
+def traffic_profile():
+    if os.path.exists(TRAFFIC_PROFILE):
+        with open(TRAFFIC_PROFILE) as f:
+            return json.load(f)
+    return None
+
+
+def quotient_roofline(stats, X, prof):
+    """k_quot over the timed steps, against VALU issue and DRAM bandwidth.
+    issue: lower-bound wave instructions per launch (positions of each kind x
+    the inner loop's instructions per position / 64 lanes) / mean launch time
+    (HIP events around the launch on the engine's stream) against 1024 SIMDs
+    issuing one wave64 instruction per 4 cycles.  dram: bytes per launch from
+    the committed counter profile (TCC_EA0_RDREQ_DRAM_32B and
+    TCC_EA0_WRREQ_WRITE_DRAM_32B, 32-byte units, calibrated on a device copy)
+    / the same launch time against 8 TB/s.  The larger fraction binds."""
     st = stats[-1]
-    byts = (BYTES_S1_DENSE * (st.s1_terms - st.s1_comp_terms) + BYTES_S1_COMP * st.s1_comp_terms
-            + BYTES_S2 * st.s2_terms)
+    dense = st.s1_terms - st.s1_comp_terms
+    insts = (VALU_PER_S1_DENSE * dense + VALU_PER_S1_COMP * st.s1_comp_terms + VALU_PER_S2_MIN * st.s2_terms) / 64.0
     ms = sum(s.quotient_ms for s in stats) / len(stats)
     sec = ms / 1000.0
-    ach = byts / sec / 1e9
-    traffic, issue = None, {}
-    tf = os.path.join(ROOT, "profiles", "traffic_k_quot_1e17.json")
-    if os.path.exists(tf):
-        prof = json.load(open(tf))
-        traffic = prof.get("hbm_bytes_per_launch")
-        issue = prof.get("issue", {}).get("k_quot", {})
-    return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_quot",
-            "algorithmic_bytes": byts, "kernel_ms": round(ms, 3), "launches": len(stats),
-            "valu_issue_frac": issue.get("valu_issue_frac"), "valu_busy": issue.get("valu_busy"),
-            "l2_hit_rate": issue.get("l2_hit_rate"),
-            "note": "per-unit bytes count every compressed lookup as a 20-byte HBM fetch; L2/MALL serve part of "
-                    "them (traffic, from FETCH/WRITE_SIZE, is below the algorithmic bytes), so frac can exceed 1; "
-                    "the binding resource is VALU issue: valu_busy (SQ_ACTIVE_INST_VALU over the SIMDs' cycles) is the measured "
-                    "fraction of SIMD cycles spent issuing VALU work, valu_issue_frac the same from SQ_INSTS_VALU at 4 cycles each"}
+    issue_ach = insts / sec / 1e9
+    issue = {"achieved": round(issue_ach, 2), "peak": ISSUE_PEAK_GINST, "unit": "Ginst/s",
+             "frac": round(issue_ach / ISSUE_PEAK_GINST, 4), "insts_per_launch": insts,
+             "per_position": {"s1_dense": VALU_PER_S1_DENSE, "s1_comp": VALU_PER_S1_COMP, "s2_min": VALU_PER_S2_MIN}}
+    dram = None
+    k = (prof or {}).get("kernels", {}).get("k_quot") if prof and prof.get("X") == X else None
+    if k:
+        byts = k["dram_read_bytes"] + k["dram_write_bytes"]
+        ach = byts / sec / 1e9
+        dram = {"achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "bytes": byts, "l2_hit_rate": k.get("l2_hit_rate")}
+    binding = "valu" if dram is None or issue["frac"] >= dram["frac"] else "hbm"
+    top = issue if binding == "valu" else dram
+    return {"bound": binding, "achieved": top["achieved"], "peak": top["peak"], "unit": top["unit"],
+            "frac": top["frac"], "traffic": dram["bytes"] if dram else None, "kernel": "k_quot",
+            "kernel_ms": round(ms, 3), "launches": len(stats), "issue": issue, "hbm": dram,
+            "valu_busy_counter": (k or {}).get("valu_busy"),
+            "algorithmic_bytes_gather_model": st.quotient_bytes,
+            "note": "issue = lower-bound VALU instructions from the inner loops / live kernel time; hbm = DRAM-side "
+                    "bytes by counters (an upper bound on HBM bytes: the MALL may serve part) / live kernel time; "
+                    "valu_busy_counter = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x cycles) from the same profile"}
+
+
+def table_roofline(stats, X, prof):
+    """Table build (k_sieve + scan + fix-up) over the timed steps: table bytes
+    written / build time against HBM, and the sieve kernel's VALU busy from
+    the committed counter profile (the binding resource)."""
+    st = stats[-1]
+    ms = sum(s.table_ms for s in stats) / len(stats)
+    ach = st.table_bytes / (ms / 1000) / 1e9
+    k = (prof or {}).get("kernels", {}).get("k_sieve") if prof and prof.get("X") == X else None
+    out = {"bound": "valu" if k else "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(ach / HBM_PEAK_GBS, 4), "bytes": st.table_bytes, "ms": round(ms, 3)}
+    if k:
+        out.update({"frac": k.get("valu_busy"), "achieved": None, "peak": None, "unit": "VALU busy",
+                    "hbm_frac": round(ach / HBM_PEAK_GBS, 4), "hbm_achieved_gbs": round(ach, 2),
+                    "valu_insts_per_mark": k.get("valu_lane_insts_per_mark"), "marks": k.get("marks"),
+                    "dram_bytes": k["dram_read_bytes"] + k["dram_write_bytes"]})
+    return out
+
+
+# ---- GPU legs ---------------------------------------------------------------------
+def config_times(expect):
+    """M(1e12) dense (C1) and M(1e15) streaming (C2) with the reference's
+    parameters and the static partition: cold (fresh context, first call) and
+    warm (median of 3, table rebuilt every call), as wall time and M(X)/s."""
+    rows = {}
+    for name, X, strat in (("C1_1e12_dense", 10**12, mdr_hip.STRATEGY_DENSE),
+                           ("C2_1e15_streaming", 10**15, mdr_hip.STRATEGY_STREAMING)):
+        o = mdr_hip.make_opts(strategy=strat, partition=mdr_hip.PARTITION_STATIC, no_table_reuse=1)
+        t0 = time.perf_counter()
+        with mdr_hip.Engine(0) as e:
+            r = e.mertens(X, o)
+            t_cold = time.perf_counter() - t0
+            ts, st = [], None
+            for _ in range(3):
+                t1 = time.perf_counter()
+                assert e.mertens(X, o) == r
+                ts.append(time.perf_counter() - t1)
+                st = e.last_stats
+        if X in expect:
+            assert r == expect[X], (X, r)
+        warm = statistics.median(ts)
+        rows[name] = {"X": X, "result": r, "cold_s": round(t_cold, 4), "warm_s_median3": round(warm, 5),
+                      "evals_per_s": round(1.0 / warm, 2), "U": st.U, "cx": st.cx,
+                      "table_ms": round(st.table_ms, 3), "quotient_ms": round(st.quotient_ms, 3)}
+        log("config", name, rows[name])
+    return rows
 
 
 def main():
@@ -106,6 +238,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-static", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the M(1e12) / M(1e15) configuration timings")
     ap.add_argument("--no-large", action="store_true", help="skip the 64-X batch in [1e16, 1e18]")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -114,8 +247,11 @@ def main():
         sys.exit(2)
     known = {int(k): v for k, v in load_json("known.json").items()}
     ref = {int(k): v for k, v in load_json("ref_values.json").items()}
+    expect_all = dict(ref)
+    expect_all.update(known)
     X = args.x
-    expect = known.get(X, ref.get(X))
+    expect = expect_all.get(X)
+    prof = traffic_profile()
 
     # cold: a fresh context's first call (device buffers allocated, sieving
     # primes uploaded, table built); the timed steps below are warm
@@ -128,20 +264,10 @@ def main():
     log(f"cold: context + first M({X}) in {t_cold:.3f} s")
     E.close()
     E = mdr_hip.Engine(0)
+    tuned = dict(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1)
     # production path: learned cuts fed to the dynamic chunk queue
-    learned = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
-                                sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
-    # partition comparison under the static round-robin schedule (one chunk per wave)
+    learned = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, sched=mdr_hip.SCHED_DYNAMIC, **tuned)
     nchunk = E.waves()
-    learned_rr = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
-                                   sched=mdr_hip.SCHED_STATIC, chunk=nchunk, no_table_reuse=1)
-    static_rr = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
-                                  sched=mdr_hip.SCHED_STATIC, chunk=nchunk, no_table_reuse=1)
-    static_q = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_STATIC,
-                                 sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
-    # the reference GPU engine's partition: whole terms (G evaluations) per chunk
-    terms_q = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_TERMS,
-                                sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
 
     def timed(opts, k, tag):
         res, stats = [], []
@@ -162,25 +288,45 @@ def main():
         log(f"warmup {i} done")
     t_learned, st_l = timed(learned, args.steps, "learned")
     used = "learned" if st_l[-1].partition_used == mdr_hip.PARTITION_LEARNED else "static (no cost model)"
+
     cmp = None
     if not args.no_static:
-        t_sq, _ = timed(static_q, args.steps, "static+queue")
-        t_lrr, _ = timed(learned_rr, args.steps, "learned+round-robin")
-        t_srr, _ = timed(static_rr, args.steps, "static+round-robin")
-        t_tq, _ = timed(terms_q, 1, "terms+queue")
+        def mode(partition, sched, chunk=0, k=args.steps, tag=""):
+            o = mdr_hip.make_opts(partition=partition, sched=sched, chunk=chunk, **tuned)
+            t, _ = timed(o, k, tag)
+            ow = mdr_hip.make_opts(partition=partition, sched=sched, chunk=chunk, wave_span=1, **tuned)
+            E.mertens(X, ow)
+            s = E.last_stats
+            return t, {"wall_s": round(t, 4), "quotient_ms": round(s.quotient_ms, 2),
+                       "wave_busy_frac": round(s.wave_busy_frac, 4), "wave_span_ms": round(s.wave_span_ms, 2),
+                       "wave_busy_ms": round(s.wave_busy_ms, 2)}
+        t_sq, m_sq = mode(mdr_hip.PARTITION_STATIC, mdr_hip.SCHED_DYNAMIC, tag="static+queue")
+        _, m_lq = mode(mdr_hip.PARTITION_LEARNED, mdr_hip.SCHED_DYNAMIC, k=1, tag="learned+queue (stamped)")
+        t_lrr, m_lrr = mode(mdr_hip.PARTITION_LEARNED, mdr_hip.SCHED_STATIC, nchunk, tag="learned+round-robin")
+        t_srr, m_srr = mode(mdr_hip.PARTITION_STATIC, mdr_hip.SCHED_STATIC, nchunk, tag="static+round-robin")
+        t_tq, m_tq = mode(mdr_hip.PARTITION_TERMS, mdr_hip.SCHED_DYNAMIC, k=1, tag="terms+queue")
+        m_lq["wall_s"] = round(t_learned, 4)
         cmp = {"schedule_round_robin_chunks": nchunk,
-               "wall_s_terms_queue": round(t_tq, 4), "terms_over_learned_queue": round(t_tq / t_learned, 3),
-               "wall_s_learned_round_robin": round(t_lrr, 4), "wall_s_static_round_robin": round(t_srr, 4),
+               "learned_queue": m_lq, "static_queue": m_sq, "learned_round_robin": m_lrr,
+               "static_round_robin": m_srr, "terms_queue": m_tq,
+               "static_over_learned_queue": round(t_sq / t_learned, 3),
                "static_over_learned_round_robin": round(t_srr / t_lrr, 3),
-               "wall_s_static_queue": round(t_sq, 4), "static_over_learned_queue": round(t_sq / t_learned, 3)}
+               "terms_over_learned_queue": round(t_tq / t_learned, 3),
+               # waves are busy this fraction of the static+queue quotient launch:
+               # no partition of the same work can shorten the launch by more
+               "partition_gain_bound_quotient": round(1.0 / m_sq["wave_busy_frac"], 4)
+               if m_sq["wave_busy_frac"] > 0 else None}
         log("partition comparison", cmp)
+
+    configs = None if args.no_configs else config_times(expect_all)
 
     batch = None
     if not args.no_batch:
         xs = load_json("xlists.json")["batch_4096_1e9_1e14_seed3"]
-        kw = dict(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED, sched=mdr_hip.SCHED_DYNAMIC)
-        bo = mdr_hip.make_opts(no_table_reuse=1, **kw)
-        bg = mdr_hip.make_opts(no_table_reuse=1, use_graph=1, **kw)
+        kw = dict(params=mdr_hip.PARAMS_TUNED, sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
+        bo = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, **kw)
+        bg = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, use_graph=1, **kw)
+        bs = mdr_hip.make_opts(partition=mdr_hip.PARTITION_STATIC, **kw)
         E.batch(xs[:64], bo)
         E.batch(xs[:64], bg)
         t0 = time.perf_counter()
@@ -190,13 +336,17 @@ def main():
         t0 = time.perf_counter()
         outg = E.batch(xs, bg)
         tg = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        outs = E.batch(xs, bs)
+        ts = time.perf_counter() - t0
         checked = [i for i, x in enumerate(xs) if x in ref]
         bad = sum(out[i] != ref[xs[i]] for i in checked) + sum(a != b for a, b in zip(out, outg))
+        bad += sum(a != b for a, b in zip(out, outs))
         # one-at-a-time: the same X values, one independent call each (own table,
         # eager and graph launches)
         one = xs[:256]
-        so = mdr_hip.make_opts(no_table_reuse=1, **kw)
-        sg = mdr_hip.make_opts(no_table_reuse=1, use_graph=1, **kw)
+        so = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, **kw)
+        sg = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, use_graph=1, **kw)
         t0 = time.perf_counter()
         single = [E.mertens(x, so) for x in one]
         t1 = time.perf_counter() - t0
@@ -207,6 +357,7 @@ def main():
         batch = {"workload": "4096 X log-uniform in [1e9, 1e14], seed 3", "n": len(xs), "seconds": round(tb, 4),
                  "evals_per_s": round(len(xs) / tb, 1),
                  "graph_seconds": round(tg, 4), "graph_evals_per_s": round(len(xs) / tg, 1),
+                 "static_partition_seconds": round(ts, 4), "static_over_learned": round(ts / tb, 3),
                  "one_at_a_time_evals_per_s": round(len(one) / t1, 1),
                  "one_at_a_time_graph_evals_per_s": round(len(one) / t1g, 1),
                  "speedup_vs_one_at_a_time": round((len(xs) / min(tb, tg)) / (len(one) / min(t1, t1g)), 2),
@@ -219,33 +370,35 @@ def main():
     if not args.no_large:
         # 64 X log-uniform in [1e16, 1e18], seed 5: one call sharing one table
         xs = load_json("xlists.json")["large_64_1e16_1e18_seed5"]
-        lo = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, partition=mdr_hip.PARTITION_LEARNED,
-                               sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
+        kw = dict(params=mdr_hip.PARAMS_TUNED, sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
         t0 = time.perf_counter()
-        out = E.batch(xs, lo)
+        out = E.batch(xs, mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, **kw))
         tl = time.perf_counter() - t0
         lst = E.last_stats
+        t0 = time.perf_counter()
+        outs = E.batch(xs, mdr_hip.make_opts(partition=mdr_hip.PARTITION_STATIC, **kw))
+        tls = time.perf_counter() - t0
         checked = [i for i, x in enumerate(xs) if x in ref]
-        bad = sum(out[i] != ref[xs[i]] for i in checked)
+        bad = sum(out[i] != ref[xs[i]] for i in checked) + sum(a != b for a, b in zip(out, outs))
         # invariance: the median X alone, with its own table and cutoff
         mid = sorted(range(len(xs)), key=lambda i: xs[i])[len(xs) // 2]
         alone = E.mertens(xs[mid], mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1))
         bad += int(alone != out[mid])
         large = {"workload": "64 X log-uniform in [1e16, 1e18], seed 5, one batch", "n": len(xs),
-                 "seconds": round(tl, 3), "evals_per_s": round(len(xs) / tl, 2), "table_U": lst.U,
-                 "table_ms": round(lst.table_ms, 1), "quotient_ms": round(lst.quotient_ms, 1),
-                 "checked_vs_reference": len(checked), "checked_alone": 1, "mismatches": bad}
+                 "seconds": round(tl, 3), "evals_per_s": round(len(xs) / tl, 2),
+                 "static_partition_seconds": round(tls, 3), "static_over_learned": round(tls / tl, 3),
+                 "table_U": lst.U, "table_ms": round(lst.table_ms, 1), "quotient_ms": round(lst.quotient_ms, 1),
+                 "checked_vs_reference": len(checked), "checked_vs_static": len(xs), "checked_alone": 1,
+                 "mismatches": bad}
         log("large batch", large)
         assert bad == 0
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(threads)
+        cpu = cpu_baseline(cpu_threads(), expect_all)
         log("cpu", cpu)
 
     st = st_l[-1]
-    t_table = sum(s.table_ms for s in st_l) / len(st_l)
     line = {
         "metric": METRIC, "value": round(1.0 / t_learned, 5), "unit": "M(X)/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_learned * 1000, 2),
@@ -266,11 +419,9 @@ def main():
         "cost_model": {"predicted_cycles": st.predicted_cycles, "measured_wave_cycles": st.measured_cycles,
                        "predicted_over_measured": (round(st.predicted_cycles / st.measured_cycles, 3)
                                                    if st.measured_cycles > 0 else None)},
-        "roofline": roofline(st_l),
-        "roofline_table": {"bound": "hbm", "achieved": round(st.table_bytes / (t_table / 1000) / 1e9, 2),
-                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(st.table_bytes / (t_table / 1000) / 1e9 / HBM_PEAK_GBS, 4),
-                           "bytes": st.table_bytes, "ms": round(t_table, 3)},
+        "roofline": quotient_roofline(st_l, X, prof),
+        "roofline_table": table_roofline(st_l, X, prof),
+        "configs": configs,
         "batch": batch,
         "large_batch": large,
         "cpu_baseline": cpu,
