@@ -2149,6 +2149,22 @@ struct DevTmp {
     hipError_t alloc(size_t b) { return hipMalloc(&p, b < 16 ? 16 : b); }
     template <typename T> T* as() { return static_cast<T*>(p); }
 };
+
+struct PinnedTmp {
+    void* p = nullptr;
+    ~PinnedTmp() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t alloc(size_t b) { return hipHostMalloc(&p, b < 16 ? 16 : b, hipHostMallocDefault); }
+};
+
+struct EventTmp {
+    hipEvent_t e = nullptr;
+    ~EventTmp() {
+        if (e) (void)hipEventDestroy(e);
+    }
+    hipError_t create() { return hipEventCreateWithFlags(&e, hipEventDisableTiming); }
+};
 }  // namespace
 
 extern "C" {
@@ -2276,7 +2292,9 @@ mdr_status mdr_mlp_train(mdr_ctx* c, const float* feats, const float* targets, s
         w0[MLP_B3] = (float)(mean / (double)n);
     }
     const size_t bsz = std::min<size_t>((size_t)batch, n);
-    DevTmp dw, dx, dt, buf, err, dg, dm, dv, didx, dl;
+    DevTmp dw, dx, dt, buf, err, dg, dm, dv, dl, didx[2];
+    PinnedTmp hidx[2];
+    EventTmp used[2];
     HIPT(dw.alloc(sizeof(float) * MLP_NPARAM), "alloc");
     HIPT(dx.alloc(sizeof(float) * MLP_IN * n), "alloc");
     HIPT(dt.alloc(sizeof(float) * n), "alloc");
@@ -2285,7 +2303,11 @@ mdr_status mdr_mlp_train(mdr_ctx* c, const float* feats, const float* targets, s
     HIPT(dg.alloc(sizeof(float) * MLP_NPARAM), "alloc");
     HIPT(dm.alloc(sizeof(float) * MLP_NPARAM), "alloc");
     HIPT(dv.alloc(sizeof(float) * MLP_NPARAM), "alloc");
-    HIPT(didx.alloc(sizeof(int) * n), "alloc");
+    for (int k = 0; k < 2; ++k) {
+        HIPT(didx[k].alloc(sizeof(int) * n), "alloc");
+        HIPT(hidx[k].alloc(sizeof(int) * n), "alloc");
+        HIPT(used[k].create(), "event");
+    }
     HIPT(dl.alloc(sizeof(float)), "alloc");
     HIPT(h2d(c, dw.p, w0.data(), sizeof(float) * MLP_NPARAM), "upload");
     HIPT(h2d(c, dx.p, feats, sizeof(float) * MLP_IN * n), "upload");
@@ -2294,17 +2316,21 @@ mdr_status mdr_mlp_train(mdr_ctx* c, const float* feats, const float* targets, s
     HIPT(clear_sync(c, dv.p, sizeof(float) * MLP_NPARAM), "clear");
     std::vector<int> idx(n);
     for (size_t i = 0; i < n; ++i) idx[i] = (int)i;
+    // Two pinned/device index buffers: the host shuffles epoch e + 1 while the
+    // GPU runs epoch e, and waits only before refilling the buffer of epoch e - 1.
     int step = 0;
     for (int e = 0; e < epochs; ++e) {
+        const int k = e & 1;
         std::shuffle(idx.begin(), idx.end(), rng);
-        HIPT(hipMemcpyAsync(didx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->st), "upload");
+        if (e >= 2) HIPT(hipEventSynchronize(used[k].e), "epoch");
+        std::memcpy(hidx[k].p, idx.data(), sizeof(int) * n);
+        HIPT(hipMemcpyAsync(didx[k].p, hidx[k].p, sizeof(int) * n, hipMemcpyHostToDevice, c->st), "upload");
         for (size_t b0 = 0; b0 + bsz <= n; b0 += bsz) {
-            HIPT(mlp_grad(dw.as<float>(), dx.as<float>(), dt.as<float>(), didx.as<int>() + b0, (i64)bsz,
-                          buf.as<float>(), nullptr, dg.as<float>(), nullptr, c->st),
-                 "gradient");
-            HIPT(mlp_adam(dw.as<float>(), dg.as<float>(), dm.as<float>(), dv.as<float>(), lr, ++step, c->st), "adam");
+            HIPT(mlp_train_step(dw.as<float>(), dx.as<float>(), dt.as<float>(), didx[k].as<int>() + b0, (i64)bsz,
+                                buf.as<float>(), dm.as<float>(), dv.as<float>(), lr, ++step, c->st),
+                 "train step");
         }
-        HIPT(hipStreamSynchronize(c->st), "epoch");
+        HIPT(hipEventRecord(used[k].e, c->st), "epoch");
     }
     // loss over all samples with the final weights
     HIPT(mlp_grad(dw.as<float>(), dx.as<float>(), dt.as<float>(), nullptr, (i64)n, buf.as<float>(), err.as<float>(),

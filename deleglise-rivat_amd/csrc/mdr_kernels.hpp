@@ -101,7 +101,10 @@ i64 scan_tmp_len(i64 n);
 hipError_t mlp_predict(const float* w, const float* X, i64 n, float* out, hipStream_t st);
 hipError_t mlp_grad(const float* w, const float* X, const float* T, const int* idx, i64 n, float* buf, float* err,
                     float* grad, float* loss, hipStream_t st);
-hipError_t mlp_adam(float* w, const float* g, float* m, float* v, float lr, int step, hipStream_t st);
+// one Adam step over the batch idx[0..n): forward/backward, then gradient and
+// update of every parameter in one launch
+hipError_t mlp_train_step(float* w, const float* X, const float* T, const int* idx, i64 n, float* buf, float* m,
+                          float* v, float lr, int step, hipStream_t st);
 i64 mlp_row_floats();
 
 }  // namespace mdr

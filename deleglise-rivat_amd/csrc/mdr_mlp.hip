@@ -2,11 +2,15 @@
 // (5 -> 32 -> 32 -> 1 FP32 MLP, ReLU; see mdr_mlp.hpp).
 //
 // Loss: mean squared error of log2(cycles per position) over a batch of n
-// samples.  The backward pass is split in two deterministic kernels: one
-// thread per sample stores the activations and the back-propagated errors
-// (x, h1, h2, g, d2, d1), then one thread per parameter sums its gradient
-// over the batch in sample order -- no atomics, so a gradient is a fixed
-// function of the batch.
+// samples.  The backward pass is split in two deterministic kernels: 32 lanes
+// per sample store the activations and the back-propagated errors (x, h1, h2,
+// g, d2, d1) field-major -- field f of sample b at f * n + b, so the reduction
+// reads consecutive samples with consecutive lanes -- then one
+// workgroup per parameter sums its gradient over the batch: thread t adds
+// samples t, t + 256, ... in order and a fixed LDS tree combines the 256
+// partial sums.  No atomics, so a gradient is a fixed function of the batch.
+// A training step applies Adam in the same workgroup that finished the
+// parameter's gradient (two launches per step).
 #include <hip/hip_runtime.h>
 #include "mdr_mlp.hpp"
 #include "mdr_kernels.hpp"
@@ -22,64 +26,121 @@ __global__ void k_mlp_predict(const float* w, const float* X, i64 n, float* out)
     if (b < n) out[b] = mlp_forward(w, X + b * MLP_IN);
 }
 
-__global__ void k_mlp_fwd(const float* w, const float* X, const float* T, const int* idx, i64 n, float* buf,
-                          float* err) {
-    const i64 b = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const i64 s = idx ? idx[b] : b;
-    float x[MLP_IN], z1[MLP_H], h1[MLP_H], z2[MLP_H];
+// Forward and backward pass of one training batch, hidden units on lanes: 32
+// lanes per sample (lane i owns hidden unit i of both layers, with row i and
+// column i of W2 in registers), FWD_S samples per workgroup.  The layers
+// exchange h1 and d2 through LDS; the output is a 32-lane shuffle reduction.
+// Writes fields x, h1, h2, g, d2, d1 of sample b at field * n + b.
+constexpr int FWD_T = 256;
+constexpr int FWD_S = FWD_T / MLP_H;   // 8 samples per workgroup
+
+__global__ __launch_bounds__(FWD_T) void k_mlp_fwd(const float* w, const float* X, const float* T, const int* idx,
+                                                   i64 n, float* buf, float* err) {
+    __shared__ float sh1[FWD_S][MLP_H];
+    __shared__ float sd2[FWD_S][MLP_H];
+    const int i = threadIdx.x % MLP_H, j = threadIdx.x / MLP_H;
+    const i64 b = (i64)blockIdx.x * FWD_S + j;
+    const bool valid = b < n;                 // no early return: the workgroup meets at barriers
+    const i64 s = valid ? (idx ? idx[b] : b) : 0;
+    float x[MLP_IN];
+#pragma unroll
     for (int k = 0; k < MLP_IN; ++k) x[k] = X[s * MLP_IN + k];
-    for (int i = 0; i < MLP_H; ++i) {
-        float acc = w[MLP_B1 + i];
-        for (int k = 0; k < MLP_IN; ++k) acc += w[MLP_W1 + i * MLP_IN + k] * x[k];
-        z1[i] = acc;
-        h1[i] = acc > 0.f ? acc : 0.f;
-    }
-    float out = w[MLP_B3];
-    for (int i = 0; i < MLP_H; ++i) {
-        float acc = w[MLP_B2 + i];
-        for (int k = 0; k < MLP_H; ++k) acc += w[MLP_W2 + i * MLP_H + k] * h1[k];
-        z2[i] = acc;
-        out += w[MLP_W3 + i] * (acc > 0.f ? acc : 0.f);
-    }
-    const float diff = out - T[s];
-    const float g = 2.f * diff / (float)n;
-    float* r = buf + b * ROW;
-    for (int k = 0; k < MLP_IN; ++k) r[R_X + k] = x[k];
-    for (int i = 0; i < MLP_H; ++i) {
-        r[R_H1 + i] = h1[i];
-        r[R_H2 + i] = z2[i] > 0.f ? z2[i] : 0.f;
-    }
-    r[R_G] = g;
-    float d2[MLP_H];
-    for (int i = 0; i < MLP_H; ++i) {
-        d2[i] = z2[i] > 0.f ? g * w[MLP_W3 + i] : 0.f;
-        r[R_D2 + i] = d2[i];
-    }
+    float row[MLP_H], col[MLP_H];             // W2[i][*], W2[*][i]
+#pragma unroll
     for (int k = 0; k < MLP_H; ++k) {
-        float acc = 0.f;
-        for (int i = 0; i < MLP_H; ++i) acc += d2[i] * w[MLP_W2 + i * MLP_H + k];
-        r[R_D1 + k] = z1[k] > 0.f ? acc : 0.f;
+        row[k] = w[MLP_W2 + i * MLP_H + k];
+        col[k] = w[MLP_W2 + k * MLP_H + i];
     }
-    if (err) err[b] = diff * diff;
+    float z1 = w[MLP_B1 + i];
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) z1 += w[MLP_W1 + i * MLP_IN + k] * x[k];
+    const float h1 = z1 > 0.f ? z1 : 0.f;
+    sh1[j][i] = h1;
+    __syncthreads();
+    float z2 = w[MLP_B2 + i];
+#pragma unroll
+    for (int k = 0; k < MLP_H; ++k) z2 += row[k] * sh1[j][k];
+    const float h2 = z2 > 0.f ? z2 : 0.f;
+    const float w3 = w[MLP_W3 + i];
+    float o = w3 * h2;
+#pragma unroll
+    for (int m = MLP_H / 2; m > 0; m >>= 1) o += __shfl_xor(o, m, MLP_H);
+    const float diff = (w[MLP_B3] + o) - (valid ? T[s] : 0.f);
+    const float g = 2.f * diff / (float)n;
+    const float d2 = z2 > 0.f ? g * w3 : 0.f;
+    sd2[j][i] = d2;
+    __syncthreads();
+    float d1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < MLP_H; ++k) d1 += sd2[j][k] * col[k];
+    d1 = z1 > 0.f ? d1 : 0.f;
+    if (!valid) return;
+    float* r = buf + b;   // field f at r[f * n]
+    if (i < MLP_IN) {
+        float xi = x[0];
+#pragma unroll
+        for (int k = 1; k < MLP_IN; ++k) xi = i == k ? x[k] : xi;
+        r[(R_X + i) * n] = xi;
+    }
+    r[(R_H1 + i) * n] = h1;
+    r[(R_H2 + i) * n] = h2;
+    r[(R_D2 + i) * n] = d2;
+    r[(R_D1 + i) * n] = d1;
+    if (i == 0) {
+        r[R_G * n] = g;
+        if (err) err[b] = diff * diff;
+    }
 }
 
-__global__ void k_mlp_reduce(const float* buf, i64 n, float* grad) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= MLP_NPARAM) return;
-    int a, c;   // gradient = sum_b row[a] * (c < 0 ? 1 : row[c])
+constexpr int REDUCE_T = 256;
+
+struct AdamStep {
+    float* m;
+    float* v;
+    float lr, c1, c2;   // c1 = 1 - 0.9^step, c2 = 1 - 0.999^step
+};
+
+__device__ inline void adam_update(float* w, int p, float gp, const AdamStep& o) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float mp = b1 * o.m[p] + (1.f - b1) * gp;
+    const float vp = b2 * o.v[p] + (1.f - b2) * gp * gp;
+    o.m[p] = mp;
+    o.v[p] = vp;
+    w[p] -= o.lr * (mp / o.c1) / (sqrtf(vp / o.c2) + 1e-8f);
+}
+
+// One workgroup per parameter p (grid = MLP_NPARAM).  With w non-null the
+// gradient goes straight into an Adam update of w[p] (grad may be null).
+__global__ __launch_bounds__(REDUCE_T) void k_mlp_reduce(const float* buf, i64 n, float* grad, float* w,
+                                                         AdamStep o) {
+    __shared__ float part[REDUCE_T];
+    const int p = blockIdx.x;
+    const int t = threadIdx.x;
+    int a, c;   // gradient = sum_b field_a[b] * (c < 0 ? 1 : field_c[b])
     if (p < MLP_B1) { a = R_D1 + p / MLP_IN; c = R_X + p % MLP_IN; }
     else if (p < MLP_W2) { a = R_D1 + (p - MLP_B1); c = -1; }
     else if (p < MLP_B2) { a = R_D2 + (p - MLP_W2) / MLP_H; c = R_H1 + (p - MLP_W2) % MLP_H; }
     else if (p < MLP_W3) { a = R_D2 + (p - MLP_B2); c = -1; }
     else if (p < MLP_B3) { a = R_G; c = R_H2 + (p - MLP_W3); }
     else { a = R_G; c = -1; }
+    const float* fa = buf + a * n;
     float acc = 0.f;
-    for (i64 b = 0; b < n; ++b) {
-        const float* r = buf + b * ROW;
-        acc += r[a] * (c < 0 ? 1.f : r[c]);
+    if (c < 0) {
+        for (i64 b = t; b < n; b += REDUCE_T) acc += fa[b];
+    } else {
+        const float* fc = buf + c * n;
+        for (i64 b = t; b < n; b += REDUCE_T) acc += fa[b] * fc[b];
     }
-    grad[p] = acc;
+    part[t] = acc;
+    __syncthreads();
+    for (int h = REDUCE_T / 2; h > 0; h >>= 1) {
+        if (t < h) part[t] += part[t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (grad) grad[p] = part[0];
+        if (w) adam_update(w, p, part[0], o);
+    }
 }
 
 __global__ void k_mlp_sum(const float* v, i64 n, float scale, float* out) {
@@ -95,18 +156,6 @@ __global__ void k_mlp_sum(const float* v, i64 n, float scale, float* out) {
     if (threadIdx.x == 0) *out = part[0] * scale;
 }
 
-__global__ void k_mlp_adam(float* w, const float* g, float* m, float* v, float lr, float c1, float c2) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= MLP_NPARAM) return;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float gp = g[p];
-    const float mp = b1 * m[p] + (1.f - b1) * gp;
-    const float vp = b2 * v[p] + (1.f - b2) * gp * gp;
-    m[p] = mp;
-    v[p] = vp;
-    w[p] -= lr * (mp / c1) / (sqrtf(vp / c2) + 1e-8f);
-}
-
 hipError_t mlp_predict(const float* w, const float* X, i64 n, float* out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_mlp_predict, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w, X, n, out);
@@ -116,15 +165,21 @@ hipError_t mlp_predict(const float* w, const float* X, i64 n, float* out, hipStr
 hipError_t mlp_grad(const float* w, const float* X, const float* T, const int* idx, i64 n, float* buf, float* err,
                     float* grad, float* loss, hipStream_t st) {
     if (n <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mlp_fwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w, X, T, idx, n, buf, err);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, buf, n, grad);
+    hipLaunchKernelGGL(k_mlp_fwd, dim3((unsigned)((n + FWD_S - 1) / FWD_S)), dim3(FWD_T), 0, st, w, X, T, idx, n, buf,
+                       err);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(MLP_NPARAM), dim3(REDUCE_T), 0, st, buf, n, grad, (float*)nullptr,
+                       AdamStep{});
     if (loss) hipLaunchKernelGGL(k_mlp_sum, dim3(1), dim3(256), 0, st, err, n, 1.f / (float)n, loss);
     return hipGetLastError();
 }
 
-hipError_t mlp_adam(float* w, const float* g, float* m, float* v, float lr, int step, hipStream_t st) {
-    const float c1 = 1.f - powf(0.9f, (float)step), c2 = 1.f - powf(0.999f, (float)step);
-    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, w, g, m, v, lr, c1, c2);
+hipError_t mlp_train_step(float* w, const float* X, const float* T, const int* idx, i64 n, float* buf, float* m,
+                          float* v, float lr, int step, hipStream_t st) {
+    if (n <= 0 || step < 1) return hipErrorInvalidValue;
+    const AdamStep o{m, v, lr, 1.f - powf(0.9f, (float)step), 1.f - powf(0.999f, (float)step)};
+    hipLaunchKernelGGL(k_mlp_fwd, dim3((unsigned)((n + FWD_S - 1) / FWD_S)), dim3(FWD_T), 0, st, w, X, T, idx, n, buf,
+                       (float*)nullptr);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(MLP_NPARAM), dim3(REDUCE_T), 0, st, buf, n, (float*)nullptr, w, o);
     return hipGetLastError();
 }
 
