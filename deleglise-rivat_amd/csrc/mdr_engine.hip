@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1923,12 +1924,14 @@ mdr_status mdr_table_read_M(mdr_ctx* c, long long first, long long count, int32_
     if (!c || count < 0 || count > MAX_CALL_ITEMS || (count && !out) || first < 0) return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
     if (!c->tab.valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
-    if (first + count - 1 > c->tab.U) return fail(c, MDR_ERR_INVALID, "read beyond the table");
+    if (count && (first > c->tab.U || count - 1 > c->tab.U - first))
+        return fail(c, MDR_ERR_INVALID, "read beyond the table");
     return read_M(c, c->tab.tp(), first, count, out);
 }
 
 mdr_status mdr_table_read_mu(mdr_ctx* c, long long first, long long count, int8_t* out) {
-    if (!c || count < 0 || count > MAX_CALL_ITEMS || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    if (!c || count < 0 || count > MAX_CALL_ITEMS || (count && !out) || first < 0 || first > LLONG_MAX - count)
+        return MDR_ERR_INVALID;
     std::lock_guard<std::mutex> lk(c->mu);
     if (!c->tab.valid) return fail(c, MDR_ERR_NOTABLE, "no table built");
     return read_mu(c, c->tab.tp(), first, count, out);
@@ -1981,13 +1984,13 @@ mdr_status mdr_table_copy_M(mdr_table* t, long long first, long long count, int3
     mdr_ctx* c = t->c;
     std::lock_guard<std::mutex> lk(c->mu);
     c->err.clear();
-    if (count && first + count - 1 > t->U) return fail(c, MDR_ERR_INVALID, "read beyond the table");
+    if (count && (first > t->U || count - 1 > t->U - first)) return fail(c, MDR_ERR_INVALID, "read beyond the table");
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
     return read_M(c, t->tp(), first, count, out);
 }
 
 mdr_status mdr_table_copy_mu(mdr_table* t, long long first, long long count, int8_t* out) {
-    if (!t || count < 0 || (count && !out) || first < 0) return MDR_ERR_INVALID;
+    if (!t || count < 0 || (count && !out) || first < 0 || first > LLONG_MAX - count) return MDR_ERR_INVALID;
     mdr_ctx* c = t->c;
     std::lock_guard<std::mutex> lk(c->mu);
     c->err.clear();
@@ -2033,6 +2036,8 @@ mdr_status mdr_table_copy_chunks(mdr_table* t, long long u_dense_ref, long long 
     c->err.clear();
     if (u_dense_ref < t->s.ud || u_dense_ref > t->U)
         return fail(c, MDR_ERR_INVALID, "u_dense_ref outside [the table's dense tier, U]");
+    if (first_chunk > (LLONG_MAX - u_dense_ref - 1) / 256 - count)   // values n of the chunks stay below 2^63
+        return fail(c, MDR_ERR_INVALID, "chunk range beyond 64-bit values");
     if (count == 0) return MDR_OK;
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, MDR_ERR_HIP, "hipSetDevice failed");
     const i64 stage = std::min<i64>(count, READ_STAGE / 4);   // chunks per dispatch

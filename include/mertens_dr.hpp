@@ -52,6 +52,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -314,15 +315,29 @@ inline int8_t mu_from_odd(int64 n, const int8_t* mu_odd) noexcept {
 namespace hip {
 
 // A table resident on the GPU (mdr_table_create): M(n) for n <= U, mu(n) for
-// n <= mu_limit, read or gathered into host memory on demand.  It belongs to
-// the context it was built on and keeps that context leased for its lifetime.
+// n <= mu_limit, read or gathered into host memory on demand.  A table made
+// by the constructor has an engine context of its own (created with it, shut
+// down with it), so any number of tables can be held while other calls lease
+// the pooled contexts.  DeviceTable::leased builds on a pooled context instead
+// and holds it until the table is destroyed: for tables that live within one
+// call (the table classes below copy one out and drop it).
 class DeviceTable {
 public:
-    DeviceTable(int64 U, int64 u_dense, int64 mu_limit) {
-        mdr_ctx* c = lease_.get();
-        hip_detail::check(mdr_table_create(c, U, u_dense, mu_limit, &t_), c, "mdr_table_create");
+    DeviceTable(int64 U, int64 u_dense, int64 mu_limit) : own_(new_context()), c_(own_) {
+        try {
+            create(U, u_dense, mu_limit);
+        } catch (...) {
+            mdr_shutdown(own_);
+            throw;
+        }
     }
-    ~DeviceTable() { mdr_table_free(t_); }
+    static DeviceTable leased(int64 U, int64 u_dense, int64 mu_limit) {
+        return DeviceTable(Leased{}, U, u_dense, mu_limit);
+    }
+    ~DeviceTable() {
+        mdr_table_free(t_);
+        if (own_) mdr_shutdown(own_);
+    }
     DeviceTable(const DeviceTable&) = delete;
     DeviceTable& operator=(const DeviceTable&) = delete;
 
@@ -331,16 +346,15 @@ public:
     int64 mu_limit() const { return info(2); }
     // M(first .. first + count - 1), mu(...)
     void copy_M(int64 first, int64 count, int32_t* out) const {
-        hip_detail::check(mdr_table_copy_M(t_, first, count, out), lease_.get(), "mdr_table_copy_M");
+        hip_detail::check(mdr_table_copy_M(t_, first, count, out), c_, "mdr_table_copy_M");
     }
     void copy_mu(int64 first, int64 count, int8_t* out) const {
-        hip_detail::check(mdr_table_copy_mu(t_, first, count, out), lease_.get(), "mdr_table_copy_mu");
+        hip_detail::check(mdr_table_copy_mu(t_, first, count, out), c_, "mdr_table_copy_mu");
     }
     std::vector<int32_t> get_M(const std::vector<int64>& q) const {
         std::vector<long long> in(q.begin(), q.end());
         std::vector<int32_t> out(q.size());
-        hip_detail::check(mdr_table_gather_M(t_, in.data(), in.size(), out.data()), lease_.get(),
-                          "mdr_table_gather_M");
+        hip_detail::check(mdr_table_gather_M(t_, in.data(), in.size(), out.data()), c_, "mdr_table_gather_M");
         return out;
     }
     int32_t get_M(int64 q) const { return get_M(std::vector<int64>{q})[0]; }
@@ -352,16 +366,35 @@ public:
     // the compressed tier above u_dense_ref in the reference's chunk layout
     void copy_chunks(int64 u_dense_ref, int64 first_chunk, int64 count, void* chunks, int32_t* checkpoints) const {
         hip_detail::check(mdr_table_copy_chunks(t_, u_dense_ref, first_chunk, count, chunks, checkpoints),
-                          lease_.get(), "mdr_table_copy_chunks");
+                          c_, "mdr_table_copy_chunks");
     }
 
 private:
+    struct Leased {};
+    DeviceTable(Leased, int64 U, int64 u_dense, int64 mu_limit) {
+        c_ = lease_.emplace().get();
+        create(U, u_dense, mu_limit);
+    }
+    static mdr_ctx* new_context() {
+        int dev = 0;
+        if (const char* e = std::getenv("MERTENS_HIP_DEVICE")) dev = std::atoi(e);
+        mdr_ctx* c = nullptr;
+        const mdr_status s = mdr_init(&c, dev);
+        if (s != MDR_OK)
+            throw std::runtime_error("mertens_dr (HIP): mdr_init failed with status " + std::to_string(s));
+        return c;
+    }
+    void create(int64 U, int64 u_dense, int64 mu_limit) {
+        hip_detail::check(mdr_table_create(c_, U, u_dense, mu_limit, &t_), c_, "mdr_table_create");
+    }
     int64 info(int k) const {
         long long v[3] = {0, 0, 0};
-        hip_detail::check(mdr_table_info(t_, &v[0], &v[1], &v[2]), lease_.get(), "mdr_table_info");
+        hip_detail::check(mdr_table_info(t_, &v[0], &v[1], &v[2]), c_, "mdr_table_info");
         return v[k];
     }
-    hip_detail::Lease lease_;
+    std::optional<hip_detail::Lease> lease_;   // leased tables: the pooled context
+    mdr_ctx* own_ = nullptr;                  // constructed tables: their own context
+    mdr_ctx* c_ = nullptr;
     mdr_table* t_ = nullptr;
 };
 
@@ -406,7 +439,7 @@ public:
         M.assign(static_cast<size_t>(m_limit + 1), 0);
         const int64 need_mu = retain_mu_odd ? u : mu_limit;
         const int64 U = retain_mu_odd ? u : std::max(m_limit, mu_limit);
-        const hip::DeviceTable t(U, U, need_mu);
+        const hip::DeviceTable t = hip::DeviceTable::leased(U, U, need_mu);
         hip_detail::copy_M16(t, m_limit, M.data());
         if (!retain_mu_odd) {
             t.copy_mu(0, mu_limit + 1, mu.data());
@@ -460,7 +493,7 @@ public:
         compressed_size = std::max<int64>(0, u_total - u_dense);
         num_chunks = compressed_size > 0 ? (compressed_size + 255) / 256 : 0;
         const int64 top = num_chunks ? u_dense : u_total;   // the dense tier's last value
-        const hip::DeviceTable t(u_total, u_dense, std::max<int64>(mu_limit, 0));
+        const hip::DeviceTable t = hip::DeviceTable::leased(u_total, u_dense, std::max<int64>(mu_limit, 0));
         dense_M.assign(static_cast<size_t>(top + 1), 0);
         hip_detail::copy_M16(t, top, dense_M.data());
         mu.assign(static_cast<size_t>(std::max<int64>(mu_limit, 0) + 1), 0);

@@ -9,6 +9,7 @@
 #include "mertens_dr.hpp"
 
 #include <cstdio>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -134,6 +135,17 @@ int main() {
         for (size_t i = 0; i < qs.size(); ++i) check(got[i] == ref.prefix[qs[i]], "device table gather");
         check(d.get_mu(30030) == ref.mu[30030] && d.get_mu(1048577) == 0, "device table mu");
         check(d.limit() == 1048576 && d.mu_limit() == 1048576 && d.dense_limit() <= 4096, "device table info");
+    }
+    // resident tables take no pooled context: more of them than the pool's
+    // four contexts, held while the same thread computes M(X)
+    {
+        std::vector<std::unique_ptr<hip::DeviceTable>> held;
+        for (int k = 0; k < 6; ++k) held.push_back(std::make_unique<hip::DeviceTable>(100000 + 50000 * k, 192, 1000));
+        check(DelégliseRivatEngine::compute_mertens(987654321LL) == 1697, "compute beside resident tables");
+        for (int k = 0; k < 6; ++k) {
+            const int64 q = 99999 + 50000 * k;
+            check(held[k]->get_M(q) == ref.prefix[q], "resident table after the call");
+        }
     }
     std::printf("%d failed\n", failures);
     return failures ? 1 : 0;

@@ -101,6 +101,13 @@ def test_table_object_errors(engine):
             t.gather_M([5, 1001])
         with pytest.raises(mdr_hip.MdrError):
             t.chunks(100, 0, 1)                       # below the table's own dense tier
+        # ranges whose end does not fit in 64 bits are rejected, not wrapped
+        with pytest.raises(mdr_hip.MdrError):
+            t.M(2**63 - 10, 20)
+        with pytest.raises(mdr_hip.MdrError):
+            t.mu(2**63 - 5, 10)
+        with pytest.raises(mdr_hip.MdrError):
+            t.chunks(192, 2**62, 1)
 
 
 def test_dropin_tables_cpp_program():
