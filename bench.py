@@ -85,7 +85,7 @@ def host_info():
 
 def cpu_threads():
     # the host share this process may use: OMP_NUM_THREADS when the host sets
-    # it (the GPU box gives each GPU 16 cores), else every CPU of the affinity set
+    # it (a one-GPU host share is 16 cores), else every CPU of the affinity set
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
         return int(env)
@@ -243,7 +243,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path would shard by "
-              "outer-term chunks with one 16-byte all-reduce per X, which this build does not run.", file=sys.stderr)
+              "outer-term chunks with one 16-byte all-reduce per X, which is not implemented.", file=sys.stderr)
         sys.exit(2)
     known = {int(k): v for k, v in load_json("known.json").items()}
     ref = {int(k): v for k, v in load_json("ref_values.json").items()}

@@ -3,7 +3,7 @@
 // TEST INFRASTRUCTURE ONLY. This is a plain C++20/OpenMP restatement of the
 // reference CPU engine (mertens_dr.hpp) used to check the HIP engine. Only the
 // tests, the smoke entry point and the CPU-baseline leg of the benchmark may
-// load it, and only as the checker. The product never links or calls it.
+// load it, and only to check results. The product never links or calls it.
 //
 // Every function cites the reference lines it restates. The arithmetic is
 // written from the mathematics: floor divisions are exact integer divisions

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke entry point and the
-CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+CPU-baseline leg of bench.py, to check results.  The product never imports it.
 """
 import ctypes
 import os

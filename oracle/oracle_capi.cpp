@@ -1,6 +1,6 @@
 // C entry points of the CPU oracle for the Python tests (ctypes).
 // TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke entry point and the
-// CPU-baseline leg of bench.py as the checker, never by the product.
+// CPU-baseline leg of bench.py to check results, never by the product.
 #include "mdr_oracle.hpp"
 
 #include <chrono>
