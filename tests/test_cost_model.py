@@ -68,14 +68,28 @@ def test_gpu_forward_matches_oracle(engine):
 
 @pytest.mark.gpu
 def test_gpu_gradient_matches_oracle(engine):
-    # fp32 kernels against the float64 oracle: rtol 1e-3, atol 1e-5
-    for seed in range(3):
+    # fp32 kernels against the float64 oracle: rtol 1e-3, atol 1e-5; batch
+    # sizes that fill every workgroup (2048) and that leave the last one
+    # partly empty (3001, 5: the forward kernel takes 8 samples per workgroup)
+    for seed, n in ((0, 2048), (1, 3001), (2, 5)):
         w = rand_weights(10 + seed)
-        X, T = rand_data(20 + seed, 2048)
+        X, T = rand_data(20 + seed, n)
         g, loss = engine.mlp_grad(w, X, T)
         l_ref, g_ref = mo.loss_and_grad(w, X, T)
         assert abs(loss - l_ref) <= 1e-3 * abs(l_ref) + 1e-5
         assert np.allclose(g, g_ref, rtol=1e-3, atol=1e-5)
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_is_deterministic(engine):
+    # one workgroup per parameter with a fixed summation order: the same batch
+    # gives the same bits, also for a batch that is not a multiple of 256
+    w = rand_weights(30)
+    for n in (4096, 3001):
+        X, T = rand_data(31, n)
+        g1, l1 = engine.mlp_grad(w, X, T)
+        g2, l2 = engine.mlp_grad(w, X, T)
+        assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32)) and l1 == l2
 
 
 @pytest.mark.gpu
