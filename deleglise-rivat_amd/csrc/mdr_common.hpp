@@ -27,6 +27,17 @@ typedef unsigned long long u64;
 // Stencil period lcm(16, 4, 9, 25, 7, 11): weights of 2,3,5,7,11 and the square
 // flags of 4, 9 and 25 repeat with it; 16 | period keeps phases 16-byte aligned.
 constexpr int STENCIL_PERIOD = 277200;
+// Second periodic pattern: the weights of the sieving primes 13, 17, 19 and 23
+// (the first PATTERN_PRIMES entries of the ascending sieving-prime list), with
+// period 13*17*19*23.  It is stored as PATTERN_REPEAT periods (plus one
+// segment) so that the read offset, base mod (PATTERN_REPEAT * period), is a
+// multiple of 4 for every segment base (a multiple of 16): dword reads.
+constexpr int PATTERN_PERIOD = 13 * 17 * 19 * 23;   // 96577
+constexpr int PATTERN_REPEAT = 4;
+constexpr int PATTERN_PRIMES = 4;
+constexpr unsigned pattern_byte(unsigned v) {   // v = n mod period (constexpr: host and device)
+    return (v % 13 == 0 ? 5u : 0u) + (v % 17 == 0 ? 5u : 0u) + (v % 19 == 0 ? 5u : 0u) + (v % 23 == 0 ? 5u : 0u);
+}
 // Values per compressed line; u_dense must be a multiple of it below U, and
 // every segment length is a multiple of it, so lines never straddle segments.
 constexpr int LINE_VALUES = 192;
