@@ -116,3 +116,18 @@ def test_batch_with_an_x_below_the_shared_cutoff_stays_one_table(engine, ref_val
     assert out[0] == ref_values[small]
     assert out[1] == -46758740
     assert out[2] == engine.mertens(xs[2], tuned(no_table_reuse=1))
+
+
+def test_batch_of_mixed_shapes(engine, known):
+    # duplicates, unsorted order, constants (X <= 3 and below 1), the small
+    # sieve-and-read path and summed X in one call: every entry equals its own
+    # single call, under the reference's rules and the tuned split, eager and
+    # as a graph launch; an empty batch is a no-op
+    xs = [10**15, -3, 0, 1, 2, 3, 10**15, 12345, 10**12, 10**15 + 7, 5 * 10**7, 10**12, 4, 10**13]
+    single = [engine.mertens(x) for x in xs]
+    assert single[0] == known[10**15] and single[8] == known[10**12] and single[1:4] == [0, 0, 1]
+    for kw in ({}, {"params": mdr_hip.PARAMS_TUNED}, {"params": mdr_hip.PARAMS_TUNED, "use_graph": 1},
+               {"partition": mdr_hip.PARTITION_LEARNED, "no_table_reuse": 1}):
+        print("mixed batch", kw, flush=True)
+        assert engine.batch(xs, **kw) == single, kw
+    assert engine.batch([]) == []
