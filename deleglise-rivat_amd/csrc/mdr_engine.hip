@@ -1510,6 +1510,27 @@ mdr_status mdr_plan(mdr_ctx* c, long long X, const mdr_opts* opts, int* streamin
     return MDR_OK;
 }
 
+double mdr_param_cx(long long X, const mdr_opts* opts) {
+    mdr_opts o;
+    mdr_default_opts(&o);
+    if (opts) o = *opts;
+    return ref_cx(X, o);
+}
+
+long long mdr_param_dense_cutoff(long long X, const mdr_opts* opts) {
+    mdr_opts o;
+    mdr_default_opts(&o);
+    if (opts) o = *opts;
+    return ref_dense_u(X, o);
+}
+
+long long mdr_param_streaming_cutoff(long long X, const mdr_opts* opts) {
+    mdr_opts o;
+    mdr_default_opts(&o);
+    if (opts) o = *opts;
+    return ref_stream_u(X, o, ref_cx(X, o));
+}
+
 mdr_status mdr_mertens_batch(mdr_ctx* c, const long long* X, size_t n, const mdr_opts* opts, long long* out,
                              mdr_stats* stats) {
     if (!c || (!X && n) || (!out && n)) return MDR_ERR_INVALID;

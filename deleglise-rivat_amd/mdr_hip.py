@@ -27,7 +27,8 @@ EXPORTS = ["mdr_default_opts", "mdr_init", "mdr_shutdown", "mdr_last_error", "md
            "mdr_cost_model_set", "mdr_cost_model_get", "mdr_cost_model_load", "mdr_cost_model_save",
            "mdr_cycle_log", "mdr_mlp_features", "mdr_mlp_predict", "mdr_mlp_grad", "mdr_mlp_train",
            "mdr_quot_waves", "mdr_table_create", "mdr_table_free", "mdr_table_info", "mdr_table_copy_M",
-           "mdr_table_copy_mu", "mdr_table_gather_M", "mdr_table_copy_chunks", "mdr_mertens_noalloc"]
+           "mdr_table_copy_mu", "mdr_table_gather_M", "mdr_table_copy_chunks", "mdr_mertens_noalloc",
+           "mdr_param_cx", "mdr_param_dense_cutoff", "mdr_param_streaming_cutoff"]
 
 MLP_NPARAM = 1281
 MLP_NFEAT = 5
@@ -106,6 +107,12 @@ def load():
     lib.mdr_mertens_dirichlet.restype = i32
     lib.mdr_plan.argtypes = [vp, ll, P(Opts), P(i32), P(ll), P(ll), P(ll), P(ll), P(dbl)]
     lib.mdr_plan.restype = i32
+    lib.mdr_param_cx.argtypes = [ll, P(Opts)]
+    lib.mdr_param_cx.restype = dbl
+    lib.mdr_param_dense_cutoff.argtypes = [ll, P(Opts)]
+    lib.mdr_param_dense_cutoff.restype = ll
+    lib.mdr_param_streaming_cutoff.argtypes = [ll, P(Opts)]
+    lib.mdr_param_streaming_cutoff.restype = ll
     lib.mdr_table_build.argtypes = [vp, ll, ll, ll]
     lib.mdr_table_build.restype = i32
     lib.mdr_table_read_M.argtypes = [vp, ll, ll, P(ctypes.c_int32)]

@@ -188,59 +188,24 @@ public:
     static constexpr int64 COMPRESSED_THRESHOLD = 100000000000LL;   // 10^11
     static constexpr int64 DENSE_LIMIT = 128000000LL;
 
-    // mertens_dr.hpp:914-923
+    // The reference's parameter helpers (mertens_dr.hpp:914-923, :931-962,
+    // :1401-1419): the environment read with the reference's ranges, the rule
+    // applied by the library (mdr_param_*), so the drop-in and the engine
+    // share one statement of each rule.
     static inline double choose_cx(int64 X) noexcept {
-        if (const char* e = std::getenv("MERTENS_CX")) {
-            const double v = std::atof(e);
-            if (v > 0.1 && v < 5.0) return v;
-        }
-        if (X >= 1000000000000000LL) return 1.35;
-        if (X >= 10000000000000LL) return 1.15;
-        if (X >= 100000000000LL) return 0.95;
-        return 0.70;
+        const mdr_opts o = hip_detail::opts_from_env();
+        return mdr_param_cx(X, &o);
     }
 
-    // mertens_dr.hpp:931-962
     static inline int64 choose_sieve_limit(int64 X, int threads) {
         (void)threads;
-        if (X <= 50000000LL) return X;
-        const double ll = std::log(std::max(2.0, std::log(static_cast<double>(X))));
-        double fx = 0.32;
-        if (const char* e = std::getenv("MERTENS_FX")) {
-            const double v = std::atof(e);
-            if (v >= 0.1 && v <= 3.0) fx = v;
-        }
-        int64 u = static_cast<int64>(fx * std::pow(static_cast<double>(X) / ll, 2.0 / 3.0));
-        const int64 S = isqrt(X);
-        if (u < 3 * S) u = 3 * S;
-        int64 cap = 1500000000LL;
-        if (const char* e = std::getenv("MERTENS_U_CAP")) {
-            const int64 v = static_cast<int64>(std::atoll(e));
-            if (v >= 100000000LL && v <= 4000000000LL) cap = v;
-        }
-        return u < cap ? u : cap;
+        const mdr_opts o = hip_detail::opts_from_env();
+        return mdr_param_dense_cutoff(X, &o);
     }
 
-    // mertens_dr.hpp:1401-1419
     static inline int64 choose_compressed_limit(int64 X) {
-        if (X < 50000000LL) return X;
-        const double ll = std::log(std::max(2.0, std::log(static_cast<double>(X))));
-        double fx = 0.95;
-        if (X >= 1000000000000000LL) fx = 1.25;
-        if (X >= 10000000000000000LL) fx = 1.50;
-        int64 u = static_cast<int64>(fx * std::pow(static_cast<double>(X) / ll, 2.0 / 3.0));
-        const int64 S = isqrt(X);
-        if (u < 3 * S) u = 3 * S;
-        u = ((u + 255) / 256) * 256;
-        int64 cap = 8000000000LL;
-        if (const char* e = std::getenv("MERTENS_CHMT_U_CAP")) {
-            const int64 v = std::atoll(e);
-            if (v >= 100000000LL && v <= 10000000000LL) cap = v;
-        }
-        const int64 ends = static_cast<int64>(choose_cx(X) * std::sqrt(static_cast<double>(X))) + 1000;
-        const int64 need = (S + 1000 > ends) ? S + 1000 : ends;
-        if (cap < need) cap = need;
-        return u < cap ? u : cap;
+        const mdr_opts o = hip_detail::opts_from_env();
+        return mdr_param_streaming_cutoff(X, &o);
     }
 
     // mertens_dr.hpp:1421-1428 (x86-64 behaviour)

@@ -29,6 +29,8 @@
  *   mdr_s1_sum         sum_dense_quotients<Stride> (mertens_dr.hpp:88-128) and the
  *                      compressed-tier loops (:1499-1504)
  *   mdr_g_eval         eval_single_S / eval_comb2 (mertens_dr.hpp:1081-1354)
+ *   mdr_param_*        choose_cx / choose_sieve_limit / choose_compressed_limit
+ *                      (mertens_dr.hpp:914-962, :1401-1419)
  *   mdr_plan           choose_cx / choose_sieve_limit / choose_compressed_limit and
  *                      the strategy switch (mertens_dr.hpp:914-962, :1027-1041,
  *                      :1401-1461)
@@ -209,6 +211,22 @@ long long mdr_quot_waves(mdr_ctx* ctx);
  * ctx may be NULL. */
 mdr_status mdr_plan(mdr_ctx* ctx, long long X, const mdr_opts* opts, int* streaming, long long* u,
                     long long* u_dense, long long* mu_limit, long long* N, double* cx);
+
+/* The reference's parameter rules as the engine applies them (host-only, no
+ * context).  opts may be NULL (the reference's defaults); out-of-range
+ * overrides are ignored as the reference ignores out-of-range variables.
+ *   mdr_param_cx                choose_cx (mertens_dr.hpp:914-923): opts->cx
+ *                               when 0.1 < cx < 5, else 1.35 / 1.15 / 0.95 / 0.70
+ *                               by decade of X
+ *   mdr_param_dense_cutoff      choose_sieve_limit (:931-962): the dense cutoff,
+ *                               with opts->fx (MERTENS_FX) and opts->u_cap_dense
+ *                               (MERTENS_U_CAP)
+ *   mdr_param_streaming_cutoff  choose_compressed_limit (:1401-1419): the
+ *                               streaming cutoff, with opts->u_cap
+ *                               (MERTENS_CHMT_U_CAP) and the cx above */
+double mdr_param_cx(long long X, const mdr_opts* opts);
+long long mdr_param_dense_cutoff(long long X, const mdr_opts* opts);
+long long mdr_param_streaming_cutoff(long long X, const mdr_opts* opts);
 
 /* Component entry points (parity tests).  mdr_table_build sieves n = 1..U
  * with the dense tier up to u_dense (a multiple of 192 when below U; U - u_dense
