@@ -54,6 +54,86 @@ constexpr SieveShape SIEVE_SHAPES[] = {{49152, 1024, 14}, {24576, 256, 13}, {983
                                        {49152, 256, 14},  {49152, 512, 14},  {24576, 512, 13}};
 constexpr int N_SIEVE_SHAPES = (int)(sizeof(SIEVE_SHAPES) / sizeof(SIEVE_SHAPES[0]));
 constexpr int SEG_MAX = 98304;
+// Every segment base is a multiple of SIEVE_UNIT (the shortest segment); the
+// large-prime stage of the sieve writes a base as sb * SIEVE_UNIT with sb a
+// 32-bit integer, so a table ends below SIEVE_UNIT * 2^32 (about 1.05e14).
+constexpr long long SIEVE_UNIT = 24576;
+constexpr long long SIEVE_MAX_U = SIEVE_UNIT * 4294967296LL - 1;
+// Large sieving primes below SQ_SPLIT have p^2 < 2^31: their segment offsets
+// come from a remainder mod p^2 (square hit) reduced mod p (first weight
+// hit); primes from SQ_SPLIT on take the remainder mod p and test each hit n
+// for p^2 | n.
+constexpr unsigned SQ_SPLIT = 46341;
+// Large primes below COOP_PRIME_LIMIT (about SEG/24 hits or more per segment)
+// are marked by a whole wavefront each, lanes on consecutive multiples; the
+// larger ones by one thread each.
+constexpr unsigned COOP_PRIME_LIMIT = 1024;
+static_assert((unsigned long long)COOP_PRIME_LIMIT * COOP_PRIME_LIMIT > SEG_MAX, "one square hit per thread-marked prime");
+static_assert(64ull * 256 * 256 > SEG_MAX, "one pass of square hits per wave-marked prime");
+
+// Per-prime constants of the large-prime sieve stage (p >= 256), one 16-byte
+// record, so that a segment's offsets take 32-bit multiplies instead of a
+// 64-bit division.  With m = p^2 below SQ_SPLIT and m = p from it on:
+//   pw = p | w << 24, w = bit_width(p - 1) | 1 (the byte weight);
+//   c  = (-SIEVE_UNIT) mod m, so (-b) mod m = sb * c mod m for b = sb * SIEVE_UNIT;
+//   s  = floor(c * 2^32 / m): q = mulhi(sb, s) leaves sb * c - q * m in [0, 2m)
+//        (Shoup's precomputed-quotient multiplication; m < 2^31);
+//   aux = floor(2^32 / p) below SQ_SPLIT (quotients of values below p^2 by p,
+//        low by at most 1), p^-1 mod 2^32 from SQ_SPLIT on (exact quotients n / p
+//        of multiples of p, and the divisibility test of n / p by p).
+struct alignas(16) PrimeConsts {
+    uint32_t pw, c, s, aux;
+};
+// The record array ends with PRIME_RECORD_PAD records of p = 0xffffff (above
+// every sieving root), so that a workgroup of up to 1,024 threads reading up
+// to 8 * 1,024 records past its last live record stays inside the array
+// without clamping its index.
+constexpr int PRIME_RECORD_PAD = 8192;
+constexpr PrimeConsts PRIME_RECORD_END{0xffffffu, 0u, 0u, 0u};
+constexpr PrimeConsts large_prime_consts(uint32_t p) {
+    const unsigned bw = 32u - (unsigned)__builtin_clz(p - 1u);
+    PrimeConsts r{p | ((bw | 1u) << 24), 0, 0, 0};
+    const u64 m = p < SQ_SPLIT ? (u64)p * p : (u64)p;
+    const u64 c = (m - (u64)SIEVE_UNIT % m) % m;
+    r.c = (uint32_t)c;
+    r.s = (uint32_t)((c << 32) / m);
+    if (p < SQ_SPLIT) {
+        r.aux = (uint32_t)((1ull << 32) / p);
+    } else {
+        uint32_t x = p;                                 // p * p = 1 mod 8: 3 bits
+        for (int i = 0; i < 4; ++i) x *= 2u - p * x;    // 6, 12, 24, 48 bits
+        r.aux = x;
+    }
+    return r;
+}
+
+// The arithmetic of the large-prime stage on those records (constexpr: the
+// sieve kernel and the host checks run this same code).
+constexpr uint32_t umin32(uint32_t x, uint32_t y) { return x < y ? x : y; }
+constexpr uint32_t mulhi32(uint32_t x, uint32_t y) { return (uint32_t)(((u64)x * y) >> 32); }
+// (-(b + 1)) mod m for the segment base b = sb * SIEVE_UNIT, i.e. the offset
+// of the first multiple of m at or after first = b + 1, from c = (-SIEVE_UNIT)
+// mod m and its Shoup quotient s (m < 2^31).
+constexpr uint32_t first_multiple(uint32_t sb, uint32_t c, uint32_t s, uint32_t m) {
+    uint32_t r = sb * c - mulhi32(sb, s) * m;   // (-b) mod m, or that plus m
+    r = umin32(r, r - m);                       // x - m wraps above x when x < m
+    r = r + m - 1u;
+    return umin32(r, r - m);
+}
+// r mod p for r < p^2 < 2^31, with aux = floor(2^32 / p) (the quotient
+// estimate is low by at most 1).
+constexpr uint32_t mod_below_square(uint32_t r, uint32_t p, uint32_t aux) {
+    const uint32_t x = r - mulhi32(r, aux) * p;
+    return umin32(x, x - p);
+}
+// For a multiple n of an odd p with n / p < 2^32 (n_lo = n mod 2^32,
+// pinv = p^-1 mod 2^32): p^2 divides n.  n / p = n_lo * pinv exactly, and a
+// 32-bit t is a multiple of p exactly when t * pinv (mod 2^32) times p stays
+// below 2^32.
+constexpr bool square_divides(uint32_t n_lo, uint32_t p, uint32_t pinv) {
+    const uint32_t q = n_lo * pinv;
+    return mulhi32(q * pinv, p) == 0u;
+}
 
 struct alignas(64) CLine {
     u64 pl[6];        // word w: pl[2w] = nonzero plane, pl[2w+1] = negative plane

@@ -122,8 +122,8 @@ struct mdr_ctx {
     DevBuf scan_tmp;
     // sieving primes
     i64 primes_root = -1;
-    int n_w = 0, n_small = 0, n_sq = 0, n_sq_small = 0;
-    DevBuf stencil, pattern, wprimes, wrcp, wweights, sqprimes;
+    int n_w = 0, n_small = 0, n_split = 0, n_coop = 0, n_sq = 0, n_sq_small = 0;
+    DevBuf stencil, pattern, wprimes, wrcp, wweights, wlarge, sqprimes;
     // windows of the compressed tier (windowed tables): lines, segment sums,
     // scan scratch, [carry M(window start - 1), window total]
     DevBuf wcl, wseg, wscan, wcarry;
@@ -351,6 +351,7 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
     std::vector<uint32_t> wp, sq;
     std::vector<double> wr;
     std::vector<uint8_t> ww;
+    std::vector<PrimeConsts> wl;
     for (i64 i = 2; i <= root; ++i) {
         if (comp[(size_t)i]) continue;
         for (i64 j = i * i; j <= root; j += i) comp[(size_t)j] = 1;
@@ -359,23 +360,29 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
             wr.push_back(1.0 / (double)i);
             const unsigned bw = 64u - (unsigned)__builtin_clzll((unsigned long long)(i - 1));
             ww.push_back((uint8_t)(bw | 1u));
+            if (i >= 256) wl.push_back(large_prime_consts((uint32_t)i));
         }
         if (i >= 7) sq.push_back((uint32_t)i);
     }
     c->n_w = (int)wp.size();
     c->n_small = (int)(std::lower_bound(wp.begin(), wp.end(), 256u) - wp.begin());
+    c->n_split = (int)(std::lower_bound(wp.begin(), wp.end(), SQ_SPLIT) - wp.begin());
+    c->n_coop = (int)(std::lower_bound(wp.begin(), wp.end(), COOP_PRIME_LIMIT) - wp.begin());
+    wl.insert(wl.end(), PRIME_RECORD_PAD, PRIME_RECORD_END);
     c->n_sq = (int)sq.size();
     // squares of p < 256 are flagged block-cooperatively; larger ones with the weights
     c->n_sq_small = (int)(std::lower_bound(sq.begin(), sq.end(), 256u) - sq.begin());
     HIPT(c->wprimes.ensure(wp.size() * 4 + 4), "alloc primes");
     HIPT(c->wrcp.ensure(wr.size() * 8 + 8), "alloc primes");
     HIPT(c->wweights.ensure(ww.size() + 4), "alloc primes");
+    HIPT(c->wlarge.ensure(wl.size() * sizeof(PrimeConsts)), "alloc primes");
     HIPT(c->sqprimes.ensure(sq.size() * 4 + 4), "alloc primes");
     if (!wp.empty()) {
         HIPT(h2d(c, c->wprimes.p, wp.data(), wp.size() * 4), "upload primes");
         HIPT(h2d(c, c->wrcp.p, wr.data(), wr.size() * 8), "upload primes");
         HIPT(h2d(c, c->wweights.p, ww.data(), ww.size()), "upload primes");
     }
+    HIPT(h2d(c, c->wlarge.p, wl.data(), wl.size() * sizeof(PrimeConsts)), "upload primes");
     if (!sq.empty()) HIPT(h2d(c, c->sqprimes.p, sq.data(), sq.size() * 4), "upload primes");
     if (!c->stencil.p) {
         // value for n with n = x+1 (mod period): weights of 2,3,5,7,11 and the
@@ -424,6 +431,7 @@ mdr_status prepare_table(mdr_ctx* c, TableStore& t, i64 U, i64 ud, i64 mu_lim, b
     if (mu_lim < 1) mu_lim = 1;
     if (U - ud > MAX_COMPRESSED_VALUES)
         return fail(c, MDR_ERR_RANGE, "table above the compressed-tier limit (U - u_dense > 192 * (2^32 - 1))");
+    if (U > SIEVE_MAX_U) return fail(c, MDR_ERR_RANGE, "table above the sieve's limit (U >= 24576 * 2^32)");
     if (shape < 0 || shape >= N_SIEVE_SHAPES) return fail(c, MDR_ERR_INVALID, "sieve shape out of range");
     const i64 seg = SIEVE_SHAPES[shape].seg;
     j = TableJob{};
@@ -470,6 +478,9 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     a.wweights = c->wweights.as<uint8_t>();
     a.n_w = c->n_w;
     a.n_small = c->n_small;
+    a.wlarge = c->wlarge.as<PrimeConsts>();
+    a.n_split = c->n_split;
+    a.n_coop = c->n_coop;
     a.sqprimes = c->sqprimes.as<uint32_t>();
     a.n_sq = c->n_sq;
     a.n_sq_small = c->n_sq_small;
@@ -632,6 +643,7 @@ struct WindowPlan {
 };
 
 mdr_status prepare_windows(mdr_ctx* c, const WindowPlan& w) {
+    if (w.U > SIEVE_MAX_U) return fail(c, MDR_ERR_RANGE, "windows above the sieve's limit (U >= 24576 * 2^32)");
     mdr_status st = ensure_primes(c, w.U);   // the windows sieve up to sqrt(U)
     if (st != MDR_OK) return st;
     const i64 seg = SIEVE_SHAPES[w.shape].seg;
@@ -679,6 +691,9 @@ mdr_status enqueue_window_table(mdr_ctx* c, const WindowPlan& w, i64 L, i64 R, c
     a.wweights = c->wweights.as<uint8_t>();
     a.n_w = c->n_w;
     a.n_small = c->n_small;
+    a.wlarge = c->wlarge.as<PrimeConsts>();
+    a.n_split = c->n_split;
+    a.n_coop = c->n_coop;
     a.sqprimes = c->sqprimes.as<uint32_t>();
     a.n_sq = c->n_sq;
     a.n_sq_small = c->n_sq_small;
@@ -1241,10 +1256,12 @@ __global__ void k_na_pattern(uint8_t* pt, i64 n) {
 }
 
 // The sieving primes up to root (< 65536) in the order ensure_primes uploads
-// them: primes >= 13 with 1/p and their byte weights, primes >= 7 for the
-// square flags.  One workgroup: an LDS sieve of root + 1 flags, then an
-// ordered compaction, 1024 numbers per step.
-__global__ __launch_bounds__(1024) void k_na_primes(int root, uint32_t* wp, double* wr, uint8_t* ww, uint32_t* sq) {
+// them: primes >= 13 with 1/p and their byte weights (and from 256 on, at
+// index k - n_small, their large-prime constants, followed by PRIME_RECORD_PAD
+// end records), primes >= 7 for the square flags.  One workgroup: an LDS
+// sieve of root + 1 flags, then an ordered compaction, 1024 numbers per step.
+__global__ __launch_bounds__(1024) void k_na_primes(int root, int n_small, uint32_t* wp, double* wr, uint8_t* ww,
+                                                    PrimeConsts* wl, uint32_t* sq) {
     __shared__ uint8_t comp[NA_MAX_ROOT + 1];
     __shared__ int cnt_w[1024 / 64], cnt_s[1024 / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1278,12 +1295,14 @@ __global__ __launch_bounds__(1024) void k_na_primes(int root, uint32_t* wp, doub
             wp[k] = (uint32_t)n;
             wr[k] = 1.0 / (double)n;
             ww[k] = (uint8_t)((32u - (uint32_t)__clz((uint32_t)n - 1u)) | 1u);
+            if (n >= 256) wl[k - n_small] = large_prime_consts((uint32_t)n);
         }
         if (in_s) sq[os + __popcll(ms & below)] = (uint32_t)n;
         base_w += tw;
         base_s += ts;
         __syncthreads();
     }
+    for (int i = tid; i < PRIME_RECORD_PAD; i += 1024) wl[(base_w > n_small ? base_w - n_small : 0) + i] = PRIME_RECORD_END;
 }
 
 __global__ void k_na_xparam(XParam xp, XParam* dst) {
@@ -1301,7 +1320,8 @@ __global__ void k_na_read(TableParam tp, i64 q, u64* out128) {
 
 // Byte offsets of one call's arrays in the scratch array.
 struct NaLayout {
-    size_t stencil = 0, pattern = 0, wp = 0, wr = 0, ww = 0, sq = 0, Md = 0, cl = 0, mu6 = 0, segsum = 0, scan = 0;
+    size_t stencil = 0, pattern = 0, wp = 0, wr = 0, ww = 0, wl = 0, sq = 0, Md = 0, cl = 0, mu6 = 0, segsum = 0,
+           scan = 0;
     size_t xparam = 0, len = 0, bounds = 0, ctr = 0, out = 0, end = 0;
 };
 
@@ -1312,13 +1332,18 @@ size_t na_take(size_t& at, size_t bytes) {
 }
 
 // Counts of the sieving primes up to root, as the kernel lays them out.
-void na_prime_counts(i64 root, int& n_w, int& n_small, int& n_sq, int& n_sq_small) {
+void na_prime_counts(i64 root, int& n_w, int& n_small, int& n_split, int& n_coop, int& n_sq, int& n_sq_small) {
     uint64_t comp[(NA_MAX_ROOT + 64) / 64] = {};
-    n_w = n_small = n_sq = n_sq_small = 0;
+    n_w = n_small = n_split = n_coop = n_sq = n_sq_small = 0;
     for (i64 i = 2; i <= root; ++i) {
         if (comp[i >> 6] >> (i & 63) & 1) continue;
         for (i64 m = i * i; m <= root; m += i) comp[m >> 6] |= 1ull << (m & 63);
-        if (i >= 13) { ++n_w; if (i < 256) ++n_small; }
+        if (i >= 13) {
+            ++n_w;
+            if (i < 256) ++n_small;
+            if (i < SQ_SPLIT) ++n_split;
+            if (i < COOP_PRIME_LIMIT) ++n_coop;
+        }
         if (i >= 7) { ++n_sq; if (i < 256) ++n_sq_small; }
     }
 }
@@ -1347,6 +1372,7 @@ bool na_layout(NaPlan& q, i64 nwaves) {
     l.wp = na_take(at, (size_t)root * 4 + 4);
     l.wr = na_take(at, (size_t)root * 8 + 8);
     l.ww = na_take(at, (size_t)root + 4);
+    l.wl = na_take(at, ((size_t)root / 2 + PRIME_RECORD_PAD) * sizeof(PrimeConsts));   // pi(root) < root / 2
     l.sq = na_take(at, (size_t)root * 4 + 4);
     l.Md = na_take(at, (size_t)(q.ud + 1) * 2);
     l.cl = na_take(at, (size_t)q.L * sizeof(CLine));
@@ -1472,7 +1498,7 @@ void mdr_shutdown(mdr_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
     DevBuf* bufs[] = {&c->tab.Md, &c->tab.cl, &c->tab.mu6, &c->tab.segsum, &c->scan_tmp, &c->stencil, &c->pattern,
-                      &c->wprimes, &c->wrcp, &c->wweights, &c->sqprimes, &c->xparam, &c->slot_len, &c->slot_cost, &c->bounds,
+                      &c->wprimes, &c->wrcp, &c->wweights, &c->wlarge, &c->sqprimes, &c->xparam, &c->slot_len, &c->slot_cost, &c->bounds,
                       &c->out128, &c->ctr, &c->stamps, &c->comp_out, &c->lookup, &c->mlp,
                       &c->wcl, &c->wseg, &c->wscan, &c->wcarry, &c->big, &c->wspan};
     for (auto& g : c->graph_exec)
@@ -1580,14 +1606,14 @@ mdr_status mdr_mertens_noalloc(mdr_ctx* c, long long X, long long* out, mdr_stat
     // stencil and sieving primes, generated on the device
     const i64 nst = (i64)STENCIL_PERIOD + SEG_MAX;
     const i64 root = isqrt_host(q.U) + 1;
-    int n_w = 0, n_small = 0, n_sq = 0, n_sq_small = 0;
-    na_prime_counts(root, n_w, n_small, n_sq, n_sq_small);
+    int n_w = 0, n_small = 0, n_split = 0, n_coop = 0, n_sq = 0, n_sq_small = 0;
+    na_prime_counts(root, n_w, n_small, n_split, n_coop, n_sq, n_sq_small);
     HIPT(hipEventRecord(c->ev[0], c->st), "event");
     hipLaunchKernelGGL(k_na_stencil, dim3((unsigned)((nst + 255) / 256)), dim3(256), 0, c->st, sc + l.stencil, nst);
     const i64 npt = (i64)PATTERN_REPEAT * PATTERN_PERIOD + SEG_MAX;
     hipLaunchKernelGGL(k_na_pattern, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, c->st, sc + l.pattern, npt);
-    hipLaunchKernelGGL(k_na_primes, dim3(1), dim3(1024), 0, c->st, (int)root, (uint32_t*)(sc + l.wp),
-                       (double*)(sc + l.wr), sc + l.ww, (uint32_t*)(sc + l.sq));
+    hipLaunchKernelGGL(k_na_primes, dim3(1), dim3(1024), 0, c->st, (int)root, n_small, (uint32_t*)(sc + l.wp),
+                       (double*)(sc + l.wr), sc + l.ww, (PrimeConsts*)(sc + l.wl), (uint32_t*)(sc + l.sq));
     HIPT(hipGetLastError(), "launch prime setup");
     // the table
     SieveArgs a;
@@ -1603,6 +1629,9 @@ mdr_status mdr_mertens_noalloc(mdr_ctx* c, long long X, long long* out, mdr_stat
     a.wweights = sc + l.ww;
     a.n_w = n_w;
     a.n_small = n_small;
+    a.wlarge = (const PrimeConsts*)(sc + l.wl);
+    a.n_split = n_split;
+    a.n_coop = n_coop;
     a.sqprimes = (const uint32_t*)(sc + l.sq);
     a.n_sq = n_sq;
     a.n_sq_small = n_sq_small;

@@ -17,6 +17,9 @@ struct SieveArgs {
     const double* wrcp;        // 1/p
     const uint8_t* wweights;   // bit_width(p-1)|1
     int n_w, n_small;          // total, and those below 256
+    const PrimeConsts* wlarge; // wlarge[i - n_small]: constants of prime i >= 256
+    int n_split;               // first index with p >= SQ_SPLIT (n_small <= n_split <= n_w)
+    int n_coop;                // first index with p >= COOP_PRIME_LIMIT (n_small <= n_coop <= n_split)
     const uint32_t* sqprimes;  // primes >= 7 whose squares are flagged
     int n_sq, n_sq_small;      // total, and those below 256 (larger ones go with the weights)
     int16_t* Md;

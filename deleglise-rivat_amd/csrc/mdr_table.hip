@@ -115,45 +115,112 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
             atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
     }
     // 4. primes 256 <= p <= sqrt(last n), one per thread, four in flight.
-    //    r2 = first mod p^2 gives the (at most one) square hit, r2 mod p the
-    //    first weight hit; both in FP64, exact: first < 2^53, the rounded
-    //    quotient estimates are within 1/2 + 2^-20 of the true quotients, and
-    //    fma(-q, m, x) is the exact remainder (|x - q m| < m is representable).
-    //    Primes ascend, so a thread stops at the first p with p^2 beyond the
-    //    segment.
-    constexpr int UNR = 4;
-    const double dfirst = (double)first;
-    const double dlen = (double)len;
-    for (int i0 = a.n_small + tid; i0 < a.n_w; i0 += UNR * THREADS) {
-        uint32_t pp[UNR];
+    //    With the base b = sb * SIEVE_UNIT, each prime's record gives
+    //    (-b) mod m by one Shoup multiplication (m = p^2 below SQ_SPLIT, p
+    //    from it on), and (-first) mod m = ((-b) mod m - 1) mod m is an
+    //    offset in the segment.  Below SQ_SPLIT it is the first square hit
+    //    and its residue mod p the first weight hit; from SQ_SPLIT on it is
+    //    the first weight hit (at most a few per segment), and a hit at n is
+    //    a square hit when p divides n / p = n * p^-1 (mod 2^32; n / p < 2^32).
+    //    Primes ascend, so a thread stops at the first p above sqrt(last n).
+    {
+        constexpr int UNR = 4;
+        const uint32_t sb = (uint32_t)(base / (u64)SIEVE_UNIT);
+        const uint32_t ulen = (uint32_t)len;
+        const uint32_t f_lo = (uint32_t)first;
+        uint32_t proot = (uint32_t)__builtin_sqrt((double)lastn);
+        while ((u64)proot * proot > lastn) --proot;
+        while ((u64)(proot + 1) * (proot + 1) <= lastn) ++proot;
+        // walks the records from index i_begin (relative to the first large
+        // prime) while their primes stay <= bound: each thread takes UNR
+        // records per step, one from each run of THREADS records, in
+        // alternating directions (thread t takes t, 2 THREADS - 1 - t, ...,
+        // so that the small primes' many hits spread over the threads), and
+        // loads its next step's records before marking these, so the loads
+        // overlap the marking.  Primes ascend and the array ends with
+        // PRIME_RECORD_PAD records of p above every bound, so reading ahead
+        // needs no index check.
+        auto walk = [&](int i_begin, uint32_t bound, auto&& mark) {
+            const PrimeConsts* rec = a.wlarge + i_begin;
+            auto at = [&](int u) { return rec[u * THREADS + ((u & 1) ? THREADS - 1 - tid : tid)]; };
+            PrimeConsts cur[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const int i = i0 + u * THREADS;
-            pp[u] = i < a.n_w ? a.wprimes[i] : 0xffffffffu;
-        }
-        if ((u64)pp[0] * pp[0] > lastn) break;
+            for (int u = 0; u < UNR; ++u) cur[u] = at(u);
+            while ((cur[0].pw & 0xffffffu) <= bound) {   // cur[0] has the step's lowest index
+                rec += UNR * THREADS;
+                PrimeConsts nxt[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const uint32_t p = pp[u];
-            if ((u64)p * p > lastn) continue;
-            const double dp = (double)p, dp2 = dp * dp;
-            const double rp = rcp_fast(dp);
-            // nearest-integer quotient estimates leave remainders in [-m, m):
-            // one correction each
-            double r2 = __builtin_fma(-__builtin_rint(dfirst * (rp * rp)), dp2, dfirst);
-            r2 += r2 < 0.0 ? dp2 : 0.0;
-            // square hits (more than one when p^2 is below the segment length)
-            for (double xs = r2 != 0.0 ? dp2 - r2 : 0.0; xs < dlen; xs += dp2) {
-                const uint32_t x = (uint32_t)xs;
-                atomicOr(&acc32[x >> 2], 0x80u << (8 * (x & 3)));
+                for (int u = 0; u < UNR; ++u) nxt[u] = at(u);
+#pragma unroll
+                for (int u = 0; u < UNR; ++u)
+                    if ((cur[u].pw & 0xffffffu) <= bound) mark(cur[u]);
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) cur[u] = nxt[u];
             }
-            double r = __builtin_fma(-__builtin_rint(r2 * rp), dp, r2);
-            r += r < 0.0 ? dp : 0.0;
-            const uint32_t ri = (uint32_t)r;
-            const uint32_t w = (32u - (uint32_t)__clz(p - 1u)) | 1u;
-            for (uint32_t x = ri ? p - ri : 0; x < (uint32_t)len; x += p)
-                atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
+        };
+        // p < COOP_PRIME_LIMIT (many hits each): one wavefront per prime, its
+        // lanes on consecutive multiples.  Wave v takes the primes v, v + NWAVES,
+        // ... (a mix of sizes for every wave): lane l loads the record of its
+        // l-th prime and computes its offsets, then the wave walks its primes
+        // one by one with those values read from the lanes (scalars).
+        {
+            const int wv = (int)__builtin_amdgcn_readfirstlane((uint32_t)wave);
+            const int ncoop = a.n_coop - a.n_small;
+            for (int i0 = 0; i0 < ncoop; i0 += 64 * NWAVES) {
+                const int i = i0 + wv + NWAVES * lane;          // < n_coop + 64 NWAVES: inside the padded array
+                const PrimeConsts pc = a.wlarge[i];
+                const uint32_t p = pc.pw & 0xffffffu;
+                const bool live = i < ncoop && p <= proot;      // a prefix of the lanes
+                const uint32_t m = p * p;
+                uint32_t x2 = 0, x1 = 0;
+                if (live) {
+                    x2 = first_multiple(sb, pc.c, pc.s, m);
+                    x1 = mod_below_square(x2, p, pc.aux);
+                }
+                const int cnt = __popcll(__ballot(live));
+                for (int j = 0; j < cnt; ++j) {
+                    const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)p, j);
+                    const uint32_t mj = pj * pj;
+                    const uint32_t wj = (uint32_t)__builtin_amdgcn_readlane((int)pc.pw, j) >> 24;
+                    const uint32_t x2j = (uint32_t)__builtin_amdgcn_readlane((int)x2, j);
+                    const uint32_t x1j = (uint32_t)__builtin_amdgcn_readlane((int)x1, j);
+                    const uint32_t xs = x2j + (uint32_t)lane * mj;   // 64 p^2 > SEG_MAX: one pass
+                    if (xs < ulen) atomicOr(&acc32[xs >> 2], 0x80u << (8 * (xs & 3)));
+                    for (uint32_t x = x1j + (uint32_t)lane * pj; x < ulen; x += 64u * pj)
+                        atomicAdd(&acc32[x >> 2], wj << (8 * (x & 3)));
+                }
+            }
         }
+        // COOP_PRIME_LIMIT <= p < SQ_SPLIT: square hits from the remainder mod p^2
+        walk(a.n_coop - a.n_small, proot < SQ_SPLIT ? proot : SQ_SPLIT - 1u, [&](const PrimeConsts& pc) {
+            const uint32_t p = pc.pw & 0xffffffu;
+            const uint32_t m = p * p;
+            const uint32_t r = first_multiple(sb, pc.c, pc.s, m);   // p^2 > SEG_MAX: at most one square hit
+            if (r < ulen) atomicOr(&acc32[r >> 2], 0x80u << (8 * (r & 3)));
+            const uint32_t w = pc.pw >> 24;
+            for (uint32_t x = mod_below_square(r, p, pc.aux); x < ulen; x += p)
+                atomicAdd(&acc32[x >> 2], w << (8 * (x & 3)));
+        });
+        // p >= SQ_SPLIT: each hit tested for p^2 | n; from the segment length
+        // on a prime hits a segment at most once
+        walk(a.n_split - a.n_small, proot, [&](const PrimeConsts& pc) {
+            const uint32_t p = pc.pw & 0xffffffu;
+            const uint32_t w = pc.pw >> 24, pinv = pc.aux;
+            uint32_t x = first_multiple(sb, pc.c, pc.s, p);
+            if (p >= ulen) {
+                if (x < ulen) {
+                    const uint32_t sh = 8u * (x & 3u);
+                    atomicAdd(&acc32[x >> 2], w << sh);
+                    if (square_divides(f_lo + x, p, pinv)) atomicOr(&acc32[x >> 2], 0x80u << sh);
+                }
+                return;
+            }
+            for (; x < ulen; x += p) {
+                const uint32_t sh = 8u * (x & 3u);
+                atomicAdd(&acc32[x >> 2], w << sh);
+                if (square_divides(f_lo + x, p, pinv)) atomicOr(&acc32[x >> 2], 0x80u << sh);
+            }
+        });
     }
     __syncthreads();
 

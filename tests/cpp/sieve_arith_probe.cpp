@@ -1,0 +1,92 @@
+// Checks the 32-bit arithmetic of the sieve's large-prime stage on the host:
+// the records of large_prime_consts and the functions first_multiple,
+// mod_below_square and square_divides (the same constexpr code the kernel
+// runs), against plain 64/128-bit remainders, for every prime from 256 up to
+// the largest sieving prime a table below SIEVE_MAX_U needs, at segment bases
+// sb = 0, 1, 2^32 - 1 and seeded random ones.  Prints the number of checks and
+// "ok", or the first failure.
+#include "mdr_common.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+using namespace mdr;
+typedef unsigned __int128 u128;
+
+static int fail(const char* what, u64 p, u64 a, u64 got, u64 want) {
+    std::printf("FAIL %s p=%llu arg=%llu got=%llu want=%llu\n", what, p, a, got, want);
+    return 1;
+}
+
+int main(int argc, char** argv) {
+    const u64 pmax = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 10273905ull;   // isqrt(SIEVE_MAX_U) = 10273904
+    const int per_prime = argc > 2 ? std::atoi(argv[2]) : 6;
+    std::vector<uint8_t> comp(pmax + 1, 0);
+    std::mt19937_64 rng(20261204);
+    u64 checks = 0, primes = 0;
+    for (u64 p = 2; p <= pmax; ++p) {
+        if (comp[p]) continue;
+        for (u64 j = p * p; j <= pmax; j += p) comp[j] = 1;
+        if (p < 256) continue;
+        ++primes;
+        const PrimeConsts pc = large_prime_consts((uint32_t)p);
+        if ((pc.pw & 0xffffffu) != p) return fail("pw", p, 0, pc.pw & 0xffffffu, p);
+        const unsigned w = (64u - (unsigned)__builtin_clzll(p - 1)) | 1u;
+        if ((pc.pw >> 24) != w) return fail("weight", p, 0, pc.pw >> 24, w);
+        const bool below = p < SQ_SPLIT;
+        const u64 m = below ? p * p : p;
+        if (m >= (1ull << 31)) return fail("modulus", p, 0, m, 1ull << 31);
+        u64 sbs[3 + 16] = {0, 1, 0xffffffffull};
+        for (int i = 0; i < per_prime; ++i) sbs[3 + i] = rng() >> 32;
+        for (int i = 0; i < 3 + per_prime; ++i) {
+            const u64 sb = sbs[i];
+            const u128 first = (u128)sb * (u64)SIEVE_UNIT + 1;
+            const u64 want = (u64)((m - (u64)(first % m)) % m);
+            const uint32_t got = first_multiple((uint32_t)sb, pc.c, pc.s, (uint32_t)m);
+            ++checks;
+            if (got != want) return fail("first_multiple", p, sb, got, want);
+            if (below) {
+                ++checks;
+                if (mod_below_square(got, (uint32_t)p, pc.aux) != want % p)
+                    return fail("mod_below_square", p, got, mod_below_square(got, (uint32_t)p, pc.aux), want % p);
+            } else {
+                // the hit at first + got and the next ones of the segment
+                for (u64 k = 0; k < 3; ++k) {
+                    const u128 n = first + got + k * p;
+                    if (n / p >= (1ull << 32)) break;
+                    const bool sq = (u64)(n % ((u128)p * p)) == 0;
+                    ++checks;
+                    if (square_divides((uint32_t)(u64)n, (uint32_t)p, pc.aux) != sq)
+                        return fail("square_divides", p, (u64)n, !sq, sq);
+                }
+            }
+        }
+        if (below) {
+            // r = 0, p - 1, p, p^2 - 1 and random r < p^2
+            const u64 rs[5] = {0, p - 1, p, m - 1, (rng() >> 1) % m};
+            for (u64 r : rs) {
+                ++checks;
+                if (mod_below_square((uint32_t)r, (uint32_t)p, pc.aux) != r % p)
+                    return fail("mod_below_square", p, r, mod_below_square((uint32_t)r, (uint32_t)p, pc.aux), r % p);
+            }
+        } else {
+            // multiples n = t p with n / p = t < 2^32: t = j p (square), j p +- 1, the largest t
+            const u64 jmax = 0xffffffffull / p;
+            const u64 js[4] = {1, jmax, 1 + (rng() >> 1) % jmax, jmax > 2 ? jmax - 1 : 1};
+            for (u64 j : js) {
+                for (int d = -1; d <= 1; ++d) {
+                    const u64 t = j * p + (u64)(long long)d;
+                    if (t == 0 || t > 0xffffffffull) continue;
+                    const uint32_t n_lo = (uint32_t)(t * p);
+                    ++checks;
+                    if (square_divides(n_lo, (uint32_t)p, pc.aux) != (d == 0))
+                        return fail("square_divides", p, t, d != 0, d == 0);
+                }
+            }
+        }
+    }
+    std::printf("%llu primes %llu checks ok\n", primes, checks);
+    return 0;
+}
