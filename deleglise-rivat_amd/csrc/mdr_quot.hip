@@ -31,9 +31,9 @@ __device__ __forceinline__ int find_x(const XParam* xp, int nx, i64 G) {
     return lo;
 }
 
-// last index i in [0, n) with a[i] <= v (a non-decreasing, a[0] <= v)
-__device__ __forceinline__ i64 last_le(const i64* a, i64 n, i64 v) {
-    i64 lo = 0, hi = n - 1;
+// last index i in [lo, n) with a[i] <= v (a non-decreasing, a[lo] <= v)
+__device__ __forceinline__ i64 last_le(const i64* a, i64 n, i64 v, i64 lo = 0) {
+    i64 hi = n - 1;
     while (lo < hi) {
         const i64 mid = (lo + hi + 1) >> 1;
         if (a[mid] <= v) lo = mid;
@@ -44,8 +44,14 @@ __device__ __forceinline__ i64 last_le(const i64* a, i64 n, i64 v) {
 
 }  // namespace
 
-// Per-slot length (1 + positions of its segments; 0 for an empty slot) and,
-// for the learned partition, predicted cycles (rounded per cost piece).
+// Position of a slot's constant term before its segments' positions: passes
+// 0 and 1 have one, a window pass (2) none.
+__device__ __forceinline__ i64 const_positions(const TableParam& tp) { return tp.pass != 2 ? 1 : 0; }
+
+// Per-slot length (the constant's position, if the pass has one, plus the
+// positions of its segments; 0 for an empty slot, which includes every slot
+// with no lookups in a window pass) and, for the learned partition, predicted
+// cycles (rounded per cost piece).
 __global__ void k_slot_len(PartArgs a) {
     const i64 G = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 pos[3] = {0, 0, 0};
@@ -56,8 +62,7 @@ __global__ void k_slot_len(PartArgs a) {
         const int ns = frame_nseg(f);
         i64 L = 0, C = 0;
         if (ns > 0) {
-            L = 1;
-            C = SLOT_OVERHEAD_CYCLES;
+            L = const_positions(a.tp);
             for (int i = 0; i < ns; ++i) {
                 const Seg sg = frame_seg(f, xp, a.tp, i);
                 L += sg.count;
@@ -69,6 +74,7 @@ __global__ void k_slot_len(PartArgs a) {
                         p = pe;
                     }
             }
+            if (L > 0) C += SLOT_OVERHEAD_CYCLES;
         }
         a.len[G] = L;
         if (a.mlp) a.cost[G] = C;
@@ -111,7 +117,7 @@ __global__ void k_cuts(PartArgs a) {
     i64 pos = 0;
     if (Lg > 0 && local >= SLOT_OVERHEAD_CYCLES) {
         local -= SLOT_OVERHEAD_CYCLES;
-        pos = 1;
+        pos = const_positions(a.tp);
         const int x = find_x(a.xp, a.nx, G);
         const XParam xp = a.xp[x];
         const Frame f = make_frame(xp, a.tp.mu6, G - xp.slot_base);
@@ -452,7 +458,14 @@ __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
             const i64 og = uniform64(a.off[G]);
             if (og >= b1) break;
             const i64 og1 = uniform64(G + 1 < a.nslots ? a.off[G + 1] : T);
-            if (og1 <= og) continue;
+            if (og1 <= og) {
+                // an empty slot (a window pass leaves most of them empty): go
+                // straight to the slot holding position og, the last one with
+                // off <= og (after G, since off[G + 1] = og), instead of
+                // walking the empty slots one load pair at a time
+                G = uniform64(last_le(a.off, a.nslots, og, G + 1)) - 1;
+                continue;
+            }
             if (G >= x_end || cur_x < 0) {
                 if (cur_x >= 0) flush(a.out128, cur_x, acc_lo, acc_hi, lane);
                 cur_x = uniform64(find_x(a.xp, a.nx, G));
@@ -467,7 +480,7 @@ __global__ __launch_bounds__(256) void k_quot(QuotArgs a) {
             }
             const bool stamp = a.stamps && (((u64)G * 0x9E3779B97F4A7C15ull) >> 40) % (u64)a.stamp_stride == 0;
             const int ns = frame_nseg(f);
-            i64 s0 = 1;
+            i64 s0 = const_positions(a.tp);
             for (int i = 0; i < ns && s0 < hi; ++i) {
                 const Seg sg = uniform_seg(frame_seg(f, xp, a.tp, i));
                 const i64 s1 = s0 + sg.count;
