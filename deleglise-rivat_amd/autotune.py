@@ -34,13 +34,30 @@ U_CAP = 390_000_000_000   # the library's cap on a 288 GB device (45 % of memory
 LINE = 192
 
 
-def cutoff(X, fx, cx):
-    """The cutoff the library derives from fx (csrc/mdr_tune.cpp tune_lookup)."""
+def cutoff(X, fx, cx, cap=True):
+    """The cutoff the library derives from fx (csrc/mdr_tune.cpp tune_lookup),
+    capped at the one-table limit (the sweep measures one table there)."""
     ll = math.log(max(2.0, math.log(X)))
     u = int(fx * (X / ll) ** (2.0 / 3.0))
     u = max(u, 3 * math.isqrt(X), int(cx * math.sqrt(X)) + 1000 + 64)
-    u = min(u, U_CAP)
+    if cap:
+        u = min(u, U_CAP)
     return (u + LINE - 1) // LINE * LINE
+
+
+def clamp_fx(X, cfg):
+    """The largest fx (3 decimals) whose cutoff at X stays within the one-table
+    cap, when cfg's would exceed it.  The sweep measured the capped table for
+    such an fx, while the library windows every table above the cap: the
+    recorded fx then gives the library the table that was measured at 10^d."""
+    fx = cfg["fx"]
+    if cutoff(X, fx, cfg["cx"], cap=False) <= U_CAP:
+        return fx
+    ll = math.log(max(2.0, math.log(X)))
+    f = math.floor(U_CAP / (X / ll) ** (2.0 / 3.0) * 1000) / 1000
+    while f > 0.001 and cutoff(X, f, cfg["cx"], cap=False) > U_CAP:
+        f = round(f - 0.001, 3)
+    return min(fx, f)
 
 
 def log(*a):
@@ -111,6 +128,7 @@ def tune_decade(T, d, waves, quick):
         t = T.sweep(xs, cfg, "chunks", chs)
     else:   # small calls: the automatic count (it adapts to the work) wins
         t = T.cost(xs, cfg)
+    cfg["fx"] = clamp_fx(xs[0], cfg)
     log(f"decade {d}: {cfg} -> {t * 1000:.2f} ms")
     return cfg, t
 
