@@ -175,17 +175,23 @@ def quotient_roofline(stats, X, prof):
     return {"bound": binding, "achieved": top["achieved"], "peak": top["peak"], "unit": top["unit"],
             "frac": top["frac"], "traffic": dram["bytes"] if dram else None, "kernel": "k_quot",
             "kernel_ms": round(ms, 3), "launches": len(stats), "issue": issue, "hbm": dram,
-            "valu_busy_counter": (k or {}).get("valu_busy"),
+            "valu_busy_counter": (k or {}).get("valu_busy"), "valu_issue_counter": (k or {}).get("valu_issue"),
             "algorithmic_bytes_gather_model": st.quotient_bytes,
             "note": "issue = lower-bound VALU instructions from the inner loops / live kernel time; hbm = DRAM-side "
                     "bytes by counters (an upper bound on HBM bytes: the MALL may serve part) / live kernel time; "
-                    "valu_busy_counter = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x cycles) from the same profile"}
+                    "valu_busy_counter = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x cycles) and valu_issue_counter = "
+                    "SQ_INSTS_VALU x 4 / (SIMDs x cycles) from the same profile"}
 
 
 def table_roofline(stats, X, prof):
     """Table build (k_sieve + scan + fix-up) over the timed steps: table bytes
-    written / build time against HBM, and the sieve kernel's VALU busy from
-    the committed counter profile (the binding resource)."""
+    written / build time against HBM, and the sieve kernel's VALU issue from
+    the committed counter profile (the binding resource): SQ_INSTS_VALU x 4
+    cycles / (SIMDs x the kernel's cycles), the share of issue slots at one
+    wave64 instruction per 4 cycles.  The profile's cycle count (GRBM_GUI_ACTIVE
+    over the dispatch) carries about 1 % of error, so a saturated kernel can
+    read slightly above 1: frac is that share capped at 1, the reading is kept
+    beside it."""
     st = stats[-1]
     ms = sum(s.table_ms for s in stats) / len(stats)
     ach = st.table_bytes / (ms / 1000) / 1e9
@@ -193,7 +199,9 @@ def table_roofline(stats, X, prof):
     out = {"bound": "valu" if k else "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(ach / HBM_PEAK_GBS, 4), "bytes": st.table_bytes, "ms": round(ms, 3)}
     if k:
-        out.update({"frac": k.get("valu_busy"), "achieved": None, "peak": None, "unit": "VALU busy",
+        issue = k.get("valu_issue")
+        out.update({"frac": round(min(1.0, issue), 4) if issue is not None else None, "achieved": None, "peak": None,
+                    "unit": "VALU issue slots", "valu_issue_measured": issue, "valu_busy_counter": k.get("valu_busy"),
                     "hbm_frac": round(ach / HBM_PEAK_GBS, 4), "hbm_achieved_gbs": round(ach, 2),
                     "valu_insts_per_mark": k.get("valu_lane_insts_per_mark"), "marks": k.get("marks"),
                     "dram_bytes": k["dram_read_bytes"] + k["dram_write_bytes"]})
