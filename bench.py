@@ -53,6 +53,10 @@ ISSUE_PEAK_GINST = SIMDS * CLOCK_HZ / CYCLES_PER_INST / 1e9
 # so the issue time below is a lower bound).
 VALU_PER_S1_DENSE, VALU_PER_S1_COMP, VALU_PER_S2_MIN = 12.5, 31.0, 13.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_mertens_1e17.json")
+# Random 64-byte-line read rate of the device by buffer size (committed
+# summary of deleglise-rivat_amd/mdr_lineprobe): the ceiling of the quotient
+# kernel's scattered table lookups, each L2 miss a 128-byte line.
+LINE_PROFILE = os.path.join(ROOT, "profiles", "random_lines_mi355x.md")
 
 
 def log(*a):
@@ -138,6 +142,21 @@ def cpu_baseline(threads, expect):
 
 
 # ---- rooflines --------------------------------------------------------------------
+def random_line_ceiling():
+    """Best random-line rate (G lines/s) over the buffers beyond the L2s
+    (64 MiB and up) from the committed probe summary, or None."""
+    if not os.path.exists(LINE_PROFILE):
+        return None
+    best = None
+    with open(LINE_PROFILE) as f:
+        for line in f:
+            m = re.match(r"\|\s*(\d+) MiB\s*\|\s*[\d.]+\s*\|\s*([\d.]+)\s*\|", line)
+            if m and int(m.group(1)) >= 64:
+                v = float(m.group(2))
+                best = v if best is None else max(best, v)
+    return best
+
+
 def traffic_profile():
     if os.path.exists(TRAFFIC_PROFILE):
         with open(TRAFFIC_PROFILE) as f:
@@ -170,6 +189,13 @@ def quotient_roofline(stats, X, prof):
         ach = byts / sec / 1e9
         dram = {"achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "bytes": byts, "l2_hit_rate": k.get("l2_hit_rate")}
+        ceil = random_line_ceiling()
+        if ceil:
+            lines = k["dram_read_bytes"] / 128.0 / sec / 1e9
+            dram["random_lines"] = {"achieved": round(lines, 2), "peak": ceil, "unit": "G lines/s",
+                                    "frac": round(lines / ceil, 4),
+                                    "note": "128-byte L2 fills per second against the device's best random "
+                                            "64-byte-line read rate (profiles/random_lines_mi355x.md)"}
     binding = "valu" if dram is None or issue["frac"] >= dram["frac"] else "hbm"
     top = issue if binding == "valu" else dram
     return {"bound": binding, "achieved": top["achieved"], "peak": top["peak"], "unit": top["unit"],
