@@ -38,6 +38,15 @@ constexpr int PATTERN_PRIMES = 4;
 constexpr unsigned pattern_byte(unsigned v) {   // v = n mod period (constexpr: host and device)
     return (v % 13 == 0 ? 5u : 0u) + (v % 17 == 0 ? 5u : 0u) + (v % 19 == 0 ? 5u : 0u) + (v % 23 == 0 ? 5u : 0u);
 }
+// Third periodic pattern: the weights of the next sieving primes 29, 31 and
+// 37 (bit_width(p - 1) | 1 = 5, 5, 7), period 29*31*37, stored the same way
+// (PATTERN_REPEAT periods plus one segment: 4 | PATTERN3_PERIOD * 4 keeps the
+// reads dword-aligned).
+constexpr int PATTERN3_PERIOD = 29 * 31 * 37;   // 33263
+constexpr int PATTERN3_PRIMES = 3;
+constexpr unsigned pattern3_byte(unsigned v) {   // v = n mod PATTERN3_PERIOD
+    return (v % 29 == 0 ? 5u : 0u) + (v % 31 == 0 ? 5u : 0u) + (v % 37 == 0 ? 7u : 0u);
+}
 // Values per compressed line; u_dense must be a multiple of it below U, and
 // every segment length is a multiple of it, so lines never straddle segments.
 constexpr int LINE_VALUES = 192;

@@ -123,7 +123,7 @@ struct mdr_ctx {
     // sieving primes
     i64 primes_root = -1;
     int n_w = 0, n_small = 0, n_split = 0, n_coop = 0, n_sq = 0, n_sq_small = 0;
-    DevBuf stencil, pattern, wprimes, wrcp, wweights, wlarge, sqprimes;
+    DevBuf stencil, pattern, pattern3, wprimes, wrcp, wweights, wlarge, sqprimes;
     // windows of the compressed tier (windowed tables): lines, segment sums,
     // scan scratch, [carry M(window start - 1), window total]
     DevBuf wcl, wseg, wscan, wcarry;
@@ -346,7 +346,7 @@ void share_batch_cutoff(std::vector<XPlan>& plans, const mdr_opts& o) {
 // ---- sieving primes and stencil ------------------------------------------------
 mdr_status ensure_primes(mdr_ctx* c, i64 U) {
     const i64 root = isqrt_host(U) + 1;
-    if (root <= c->primes_root && c->stencil.p && c->pattern.p) return MDR_OK;
+    if (root <= c->primes_root && c->stencil.p && c->pattern.p && c->pattern3.p) return MDR_OK;
     std::vector<uint8_t> comp((size_t)root + 1, 0);
     std::vector<uint32_t> wp, sq;
     std::vector<double> wr;
@@ -405,6 +405,13 @@ mdr_status ensure_primes(mdr_ctx* c, i64 U) {
         for (size_t x = 0; x < pt.size(); ++x) pt[x] = (uint8_t)pattern_byte((unsigned)(x % PATTERN_PERIOD) + 1);
         HIPT(c->pattern.ensure(pt.size()), "alloc pattern");
         HIPT(h2d(c, c->pattern.p, pt.data(), pt.size()), "upload pattern");
+    }
+    if (!c->pattern3.p) {
+        // weights of 29, 31, 37 for n = x+1, over PATTERN_REPEAT periods and a segment
+        std::vector<uint8_t> pt((size_t)PATTERN_REPEAT * PATTERN3_PERIOD + SEG_MAX);
+        for (size_t x = 0; x < pt.size(); ++x) pt[x] = (uint8_t)pattern3_byte((unsigned)(x % PATTERN3_PERIOD) + 1);
+        HIPT(c->pattern3.ensure(pt.size()), "alloc pattern");
+        HIPT(h2d(c, c->pattern3.p, pt.data(), pt.size()), "upload pattern");
     }
     c->primes_root = root;
     return MDR_OK;
@@ -473,6 +480,7 @@ mdr_status enqueue_table(mdr_ctx* c, const TableJob& j) {
     a.mu_segs = j.musegs;
     a.stencil = c->stencil.as<uint8_t>();
     a.pattern = c->pattern.as<uint8_t>();
+    a.pattern3 = c->pattern3.as<uint8_t>();
     a.wprimes = c->wprimes.as<uint32_t>();
     a.wrcp = c->wrcp.as<double>();
     a.wweights = c->wweights.as<uint8_t>();
@@ -686,6 +694,7 @@ mdr_status enqueue_window_table(mdr_ctx* c, const WindowPlan& w, i64 L, i64 R, c
     a.mu_segs = 0;  // mu comes from the resident table
     a.stencil = c->stencil.as<uint8_t>();
     a.pattern = c->pattern.as<uint8_t>();
+    a.pattern3 = c->pattern3.as<uint8_t>();
     a.wprimes = c->wprimes.as<uint32_t>();
     a.wrcp = c->wrcp.as<double>();
     a.wweights = c->wweights.as<uint8_t>();
@@ -1255,6 +1264,12 @@ __global__ void k_na_pattern(uint8_t* pt, i64 n) {
     if (x < n) pt[x] = (uint8_t)pattern_byte((unsigned)(x % PATTERN_PERIOD) + 1);
 }
 
+// The weights of 29, 31 and 37 for n = x + 1, as ensure_primes builds them.
+__global__ void k_na_pattern3(uint8_t* pt, i64 n) {
+    const i64 x = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) pt[x] = (uint8_t)pattern3_byte((unsigned)(x % PATTERN3_PERIOD) + 1);
+}
+
 // The sieving primes up to root (< 65536) in the order ensure_primes uploads
 // them: primes >= 13 with 1/p and their byte weights (and from 256 on, at
 // index k - n_small, their large-prime constants, followed by PRIME_RECORD_PAD
@@ -1320,7 +1335,7 @@ __global__ void k_na_read(TableParam tp, i64 q, u64* out128) {
 
 // Byte offsets of one call's arrays in the scratch array.
 struct NaLayout {
-    size_t stencil = 0, pattern = 0, wp = 0, wr = 0, ww = 0, wl = 0, sq = 0, Md = 0, cl = 0, mu6 = 0, segsum = 0,
+    size_t stencil = 0, pattern = 0, pattern3 = 0, wp = 0, wr = 0, ww = 0, wl = 0, sq = 0, Md = 0, cl = 0, mu6 = 0, segsum = 0,
            scan = 0;
     size_t xparam = 0, len = 0, bounds = 0, ctr = 0, out = 0, end = 0;
 };
@@ -1369,6 +1384,7 @@ bool na_layout(NaPlan& q, i64 nwaves) {
     size_t at = 0;
     l.stencil = na_take(at, (size_t)STENCIL_PERIOD + SEG_MAX);
     l.pattern = na_take(at, (size_t)PATTERN_REPEAT * PATTERN_PERIOD + SEG_MAX);
+    l.pattern3 = na_take(at, (size_t)PATTERN_REPEAT * PATTERN3_PERIOD + SEG_MAX);
     l.wp = na_take(at, (size_t)root * 4 + 4);
     l.wr = na_take(at, (size_t)root * 8 + 8);
     l.ww = na_take(at, (size_t)root + 4);
@@ -1497,7 +1513,7 @@ void mdr_shutdown(mdr_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
-    DevBuf* bufs[] = {&c->tab.Md, &c->tab.cl, &c->tab.mu6, &c->tab.segsum, &c->scan_tmp, &c->stencil, &c->pattern,
+    DevBuf* bufs[] = {&c->tab.Md, &c->tab.cl, &c->tab.mu6, &c->tab.segsum, &c->scan_tmp, &c->stencil, &c->pattern, &c->pattern3,
                       &c->wprimes, &c->wrcp, &c->wweights, &c->wlarge, &c->sqprimes, &c->xparam, &c->slot_len, &c->slot_cost, &c->bounds,
                       &c->out128, &c->ctr, &c->stamps, &c->comp_out, &c->lookup, &c->mlp,
                       &c->wcl, &c->wseg, &c->wscan, &c->wcarry, &c->big, &c->wspan};
@@ -1612,6 +1628,9 @@ mdr_status mdr_mertens_noalloc(mdr_ctx* c, long long X, long long* out, mdr_stat
     hipLaunchKernelGGL(k_na_stencil, dim3((unsigned)((nst + 255) / 256)), dim3(256), 0, c->st, sc + l.stencil, nst);
     const i64 npt = (i64)PATTERN_REPEAT * PATTERN_PERIOD + SEG_MAX;
     hipLaunchKernelGGL(k_na_pattern, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, c->st, sc + l.pattern, npt);
+    const i64 npt3 = (i64)PATTERN_REPEAT * PATTERN3_PERIOD + SEG_MAX;
+    hipLaunchKernelGGL(k_na_pattern3, dim3((unsigned)((npt3 + 255) / 256)), dim3(256), 0, c->st, sc + l.pattern3,
+                       npt3);
     hipLaunchKernelGGL(k_na_primes, dim3(1), dim3(1024), 0, c->st, (int)root, n_small, (uint32_t*)(sc + l.wp),
                        (double*)(sc + l.wr), sc + l.ww, (PrimeConsts*)(sc + l.wl), (uint32_t*)(sc + l.sq));
     HIPT(hipGetLastError(), "launch prime setup");
@@ -1624,6 +1643,7 @@ mdr_status mdr_mertens_noalloc(mdr_ctx* c, long long X, long long* out, mdr_stat
     a.mu_segs = q.musegs;
     a.stencil = sc + l.stencil;
     a.pattern = sc + l.pattern;
+    a.pattern3 = sc + l.pattern3;
     a.wprimes = (const uint32_t*)(sc + l.wp);
     a.wrcp = (const double*)(sc + l.wr);
     a.wweights = sc + l.ww;

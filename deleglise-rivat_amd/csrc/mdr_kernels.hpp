@@ -13,6 +13,7 @@ struct SieveArgs {
     i64 mu_segs;               // segments whose packed mu is written
     const uint8_t* stencil;    // STENCIL_PERIOD + SEG bytes
     const uint8_t* pattern;    // PATTERN_REPEAT * PATTERN_PERIOD + SEG_MAX bytes (weights of 13..23)
+    const uint8_t* pattern3;   // PATTERN_REPEAT * PATTERN3_PERIOD + SEG_MAX bytes (weights of 29, 31, 37)
     const uint32_t* wprimes;   // sieving primes >= 13, ascending
     const double* wrcp;        // 1/p
     const uint8_t* wweights;   // bit_width(p-1)|1

@@ -56,19 +56,22 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
     const int len = (int)(lastn - base);
 
     // 1. periodic stencils: weights of 2,3,5,7,11 and flags of 4, 9, 25, plus
-    //    the weights of 13, 17, 19, 23 (byte sums stay below 0x80: no carries)
+    //    the weights of 13, 17, 19, 23 and of 29, 31, 37 from two more
+    //    patterns (byte sums stay below 0x80: no carries)
     {
         const uint32_t phase = (uint32_t)(base % STENCIL_PERIOD);
         const uint4* src = reinterpret_cast<const uint4*>(a.stencil + phase);
         const uint32_t* pat =
             reinterpret_cast<const uint32_t*>(a.pattern + base % ((u64)PATTERN_REPEAT * PATTERN_PERIOD));
+        const uint32_t* pat3 =
+            reinterpret_cast<const uint32_t*>(a.pattern3 + base % ((u64)PATTERN_REPEAT * PATTERN3_PERIOD));
         uint4* dst = reinterpret_cast<uint4*>(acc32);
         for (int i = tid; i < SEGT / 16; i += THREADS) {
             uint4 v = src[i];
-            v.x += pat[4 * i];
-            v.y += pat[4 * i + 1];
-            v.z += pat[4 * i + 2];
-            v.w += pat[4 * i + 3];
+            v.x += pat[4 * i] + pat3[4 * i];
+            v.y += pat[4 * i + 1] + pat3[4 * i + 1];
+            v.z += pat[4 * i + 2] + pat3[4 * i + 2];
+            v.w += pat[4 * i + 3] + pat3[4 * i + 3];
             dst[i] = v;
         }
     }
@@ -98,9 +101,10 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
         for (uint32_t x = sq_off[i] + (uint32_t)lane * p2; x < (uint32_t)len; x += 64u * p2)
             atomicOr(&acc32[x >> 2], 0x80u << (8 * (x & 3)));
     }
-    // 3. weights for 29 <= p < 256 (13..23 came with the pattern): the
+    // 3. weights for 41 <= p < 256 (13..37 came with the patterns): the
     //    block-wide loop for p < 64 (many hits per thread), the wave loop above it
-    int i3 = a.n_small < PATTERN_PRIMES ? a.n_small : PATTERN_PRIMES;
+    constexpr int NPAT = PATTERN_PRIMES + PATTERN3_PRIMES;
+    int i3 = a.n_small < NPAT ? a.n_small : NPAT;
     for (; i3 < a.n_small; ++i3) {
         const uint32_t p = a.wprimes[i3];
         if (p >= 64) break;
