@@ -129,6 +129,10 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
     //    Primes ascend, so a thread stops at the first p above sqrt(last n).
     {
         constexpr int UNR = 4;
+        // the walks read up to (2 UNR - 1) THREADS records past a live one,
+        // the wave-per-prime loop up to 64 NWAVES past the cooperative primes
+        static_assert((2 * UNR - 1) * THREADS <= PRIME_RECORD_PAD && 64 * NWAVES <= PRIME_RECORD_PAD,
+                      "record reads past the end stay inside the end records");
         const uint32_t sb = (uint32_t)(base / (u64)SIEVE_UNIT);
         const uint32_t ulen = (uint32_t)len;
         const uint32_t f_lo = (uint32_t)first;
