@@ -249,17 +249,24 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
         const bool general = first < (1ull << 17);
         const uint32_t add_lo = (127u - tseg) * 0x01010101u, add_hi = (126u - tseg) * 0x01010101u;
         uint4* acc4 = reinterpret_cast<uint4*>(acc32);
+        // byte masks from bits 0, 8, 16, 24 by shifts and a subtraction
+        // (b * 0xfe = (b << 8) - (b << 1), b * 0xff = (b << 8) - b: nothing
+        // carries between bytes) instead of 32-bit multiplies
         auto swar = [](uint32_t x, uint32_t add) {
             const uint32_t gt = ((x & 0x7f7f7f7fu) + add) & 0x80808080u;
             const uint32_t t = (x & 0x01010101u) ^ (gt >> 7);
-            return (0xffffffffu ^ (t * 0xfeu)) & ~(((x >> 7) & 0x01010101u) * 0xffu);
+            const uint32_t f = (x >> 7) & 0x01010101u;   // square flags
+            return ~((t << 8) - (t << 1)) & ~((f << 8) - f);
         };
         // bits at 0, 8, 16, 24 gathered to bits 0..3 (the four partial
         // products of the multiply land on distinct bits, so nothing carries)
+        // the nonzero bits (0, 8, 16, 24) and the negative bits (7, 15, 23,
+        // 31, moved to 4, 12, 20, 28) gathered to bits 0..3 and 4..7: a bit at
+        // 8i + j (j = 0 or 4) reaches i + j through the shift by 7i, and no
+        // other shifted bit lands in 0..7
         auto nibs = [](uint32_t mu) {
-            const uint32_t nzn = (((mu & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
-            const uint32_t ngn = ((((mu >> 7) & 0x01010101u) * 0x00204081u) >> 21) & 0xfu;
-            return nzn | (ngn << 4);
+            const uint32_t c = (mu & 0x01010101u) | ((mu >> 3) & 0x10101010u);
+            return (c | (c >> 7) | (c >> 14) | (c >> 21)) & 0xffu;
         };
         for (uint32_t g = tid; g < SEGT / 16; g += THREADS) {
             uint4 v = acc4[g];
