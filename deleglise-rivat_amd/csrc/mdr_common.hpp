@@ -144,6 +144,38 @@ constexpr bool square_divides(uint32_t n_lo, uint32_t p, uint32_t pinv) {
     return mulhi32(q * pinv, p) == 0u;
 }
 
+// Finalisation of the byte-log accumulator (constexpr: the sieve kernel and
+// the host checks run this same code).  A byte a holds the odd weights of the
+// sieving primes dividing n and 0x80 for a square factor: mu = 0 for a flag,
+// otherwise mu = +1 exactly when the parity bit differs from [a > floor(log2 n)]
+// (a remaining cofactor above the sieving bound is one more prime).
+constexpr int8_t mu_byte(uint8_t a, u64 n) {
+    if (a & 0x80) return 0;
+    const unsigned thr = 63u - (unsigned)__builtin_clzll(n);
+    const int s = (a & 1) ? -1 : 1;
+    return (int8_t)(a > thr ? s : -s);
+}
+// Four bytes at once for values sharing floor(log2 n) = thr, with
+// add = (127 - thr) * 0x01010101: (a & 0x7f) + 127 - thr sets bit 7 exactly
+// when a & 0x7f > thr and never carries into the next byte.  The byte masks
+// come from bits 0, 8, 16, 24 by shifts and a subtraction (b * 0xfe =
+// (b << 8) - (b << 1), b * 0xff = (b << 8) - b: nothing carries between
+// bytes) rather than by 32-bit multiplies.
+constexpr uint32_t mu_swar(uint32_t x, uint32_t add) {
+    const uint32_t gt = ((x & 0x7f7f7f7fu) + add) & 0x80808080u;
+    const uint32_t t = (x & 0x01010101u) ^ (gt >> 7);
+    const uint32_t f = (x >> 7) & 0x01010101u;   // square flags
+    return ~((t << 8) - (t << 1)) & ~((f << 8) - f);
+}
+// The nonzero bits (0, 8, 16, 24) and the negative bits (7, 15, 23, 31, moved
+// to 4, 12, 20, 28) of four mu bytes gathered to bits 0..3 and 4..7: a bit at
+// 8i + j (j = 0 or 4) reaches i + j through the shift by 7i, and no other
+// shifted bit lands in 0..7.
+constexpr uint32_t mu_nibbles(uint32_t mu) {
+    const uint32_t c = (mu & 0x01010101u) | ((mu >> 3) & 0x10101010u);
+    return (c | (c >> 7) | (c >> 14) | (c >> 21)) & 0xffu;
+}
+
 struct alignas(64) CLine {
     u64 pl[6];        // word w: pl[2w] = nonzero plane, pl[2w+1] = negative plane
     int32_t ck[3];    // word w: M(value before its first value)

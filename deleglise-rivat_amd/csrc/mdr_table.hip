@@ -27,12 +27,6 @@ __device__ __forceinline__ uint32_t mod_u64(u64 x, uint32_t p, double rp) {
     return (uint32_t)r;
 }
 
-__device__ __forceinline__ int8_t finalize(uint8_t a, u64 n) {
-    if (a & 0x80) return 0;
-    const unsigned thr = 63u - (unsigned)__clzll(n);
-    const int s = (a & 1) ? -1 : 1;
-    return (int8_t)(a > thr ? s : -s);
-}
 
 }  // namespace
 
@@ -249,33 +243,14 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
         const bool general = first < (1ull << 17);
         const uint32_t add_lo = (127u - tseg) * 0x01010101u, add_hi = (126u - tseg) * 0x01010101u;
         uint4* acc4 = reinterpret_cast<uint4*>(acc32);
-        // byte masks from bits 0, 8, 16, 24 by shifts and a subtraction
-        // (b * 0xfe = (b << 8) - (b << 1), b * 0xff = (b << 8) - b: nothing
-        // carries between bytes) instead of 32-bit multiplies
-        auto swar = [](uint32_t x, uint32_t add) {
-            const uint32_t gt = ((x & 0x7f7f7f7fu) + add) & 0x80808080u;
-            const uint32_t t = (x & 0x01010101u) ^ (gt >> 7);
-            const uint32_t f = (x >> 7) & 0x01010101u;   // square flags
-            return ~((t << 8) - (t << 1)) & ~((f << 8) - f);
-        };
-        // bits at 0, 8, 16, 24 gathered to bits 0..3 (the four partial
-        // products of the multiply land on distinct bits, so nothing carries)
-        // the nonzero bits (0, 8, 16, 24) and the negative bits (7, 15, 23,
-        // 31, moved to 4, 12, 20, 28) gathered to bits 0..3 and 4..7: a bit at
-        // 8i + j (j = 0 or 4) reaches i + j through the shift by 7i, and no
-        // other shifted bit lands in 0..7
-        auto nibs = [](uint32_t mu) {
-            const uint32_t c = (mu & 0x01010101u) | ((mu >> 3) & 0x10101010u);
-            return (c | (c >> 7) | (c >> 14) | (c >> 21)) & 0xffu;
-        };
         for (uint32_t g = tid; g < SEGT / 16; g += THREADS) {
             uint4 v = acc4[g];
             if (!general && g < gend && !(straddle && g == gb)) {
                 const uint32_t add = g < gb ? add_lo : add_hi;
-                v.x = swar(v.x, add);
-                v.y = swar(v.y, add);
-                v.z = swar(v.z, add);
-                v.w = swar(v.w, add);
+                v.x = mu_swar(v.x, add);
+                v.y = mu_swar(v.y, add);
+                v.z = mu_swar(v.z, add);
+                v.w = mu_swar(v.w, add);
             } else {
                 uint32_t* w = &v.x;
 #pragma unroll
@@ -286,7 +261,7 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
                         const u64 n = first + 16 * (u64)g + 4 * d + bb;
                         int8_t m = 0;
                         if (n <= (u64)a.U) {
-                            m = finalize((uint8_t)(w[d] >> (8 * bb)), n);
+                            m = mu_byte((uint8_t)(w[d] >> (8 * bb)), n);
                             if (n == 1) m = 1;
                             if (n == 2) m = -1;
                         }
@@ -296,7 +271,7 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
                 }
             }
             acc4[g] = v;
-            nib32[g] = nibs(v.x) | (nibs(v.y) << 8) | (nibs(v.z) << 16) | (nibs(v.w) << 24);
+            nib32[g] = mu_nibbles(v.x) | (mu_nibbles(v.y) << 8) | (mu_nibbles(v.z) << 16) | (mu_nibbles(v.w) << 24);
         }
     }
     __syncthreads();

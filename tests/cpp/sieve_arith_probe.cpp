@@ -1,10 +1,11 @@
-// Checks the 32-bit arithmetic of the sieve's large-prime stage on the host:
-// the records of large_prime_consts and the functions first_multiple,
-// mod_below_square and square_divides (the same constexpr code the kernel
-// runs), against plain 64/128-bit remainders, for every prime from 256 up to
-// the largest sieving prime a table below SIEVE_MAX_U needs, at segment bases
-// sb = 0, 1, 2^32 - 1 and seeded random ones.  Prints the number of checks and
-// "ok", or the first failure.
+// Checks the sieve's 32-bit arithmetic on the host, running the same constexpr
+// code as the kernel: the records of large_prime_consts and the functions
+// first_multiple, mod_below_square and square_divides against plain
+// 64/128-bit remainders, for every prime from 256 up to the largest sieving
+// prime a table below SIEVE_MAX_U needs, at segment bases sb = 0, 1, 2^32 - 1
+// and seeded random ones; and the four-byte finalisation mu_swar and
+// mu_nibbles against the per-byte rule mu_byte.  Prints the number of checks
+// and "ok", or the first failure.
 #include "mdr_common.hpp"
 
 #include <cstdio>
@@ -86,6 +87,36 @@ int main(int argc, char** argv) {
                 }
             }
         }
+    }
+    // finalisation: mu_swar against mu_byte for every byte value and every
+    // threshold (bytes are independent: check each value in every position,
+    // beside seeded other bytes), mu_nibbles against the per-bit gather
+    for (unsigned thr = 0; thr < 63; ++thr) {
+        const uint32_t add = (127u - thr) * 0x01010101u;
+        const u64 n = 1ull << thr;   // floor(log2 n) = thr
+        for (unsigned a = 0; a < 256; ++a) {
+            for (int pos = 0; pos < 4; ++pos) {
+                const uint32_t other = (uint32_t)rng();
+                const uint32_t x = (other & ~(0xffu << (8 * pos))) | (a << (8 * pos));
+                const uint32_t m = mu_swar(x, add);
+                for (int b = 0; b < 4; ++b) {
+                    const uint8_t want = (uint8_t)mu_byte((uint8_t)(x >> (8 * b)), n);
+                    ++checks;
+                    if ((uint8_t)(m >> (8 * b)) != want) return fail("mu_swar", thr, x, (m >> (8 * b)) & 0xff, want);
+                }
+            }
+        }
+    }
+    const uint8_t mus[3] = {0x00, 0x01, 0xff};
+    for (int i = 0; i < 81; ++i) {
+        uint32_t mu = 0, want = 0;
+        for (int b = 0, k = i; b < 4; ++b, k /= 3) {
+            const uint8_t v = mus[k % 3];
+            mu |= (uint32_t)v << (8 * b);
+            want |= (uint32_t)(v != 0) << b | (uint32_t)(v == 0xff) << (4 + b);
+        }
+        ++checks;
+        if (mu_nibbles(mu) != want) return fail("mu_nibbles", 0, mu, mu_nibbles(mu), want);
     }
     std::printf("%llu primes %llu checks ok\n", primes, checks);
     return 0;
