@@ -392,6 +392,8 @@ def main():
                  "evals_per_s": round(len(xs) / tb, 1),
                  "graph_seconds": round(tg, 4), "graph_evals_per_s": round(len(xs) / tg, 1),
                  "static_partition_seconds": round(ts, 4), "static_over_learned": round(ts / tb, 3),
+                 "partition_used": "learned" if bst.partition_used == mdr_hip.PARTITION_LEARNED else
+                                   "static (every X below 1e14, the cost model's training range)",
                  "one_at_a_time_evals_per_s": round(len(one) / t1, 1),
                  "one_at_a_time_graph_evals_per_s": round(len(one) / t1g, 1),
                  "speedup_vs_one_at_a_time": round((len(xs) / min(tb, tg)) / (len(one) / min(t1, t1g)), 2),

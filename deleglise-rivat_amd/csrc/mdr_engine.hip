@@ -905,7 +905,10 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
              "upload");
     }
     if (b.nxs) {
-        b.learned = o.partition == MDR_PARTITION_LEARNED && c->mlp_ok;
+        i64 xmax_sum = 0;
+        for (const XPlan& p : plans)
+            if (p.kind == 2) xmax_sum = std::max(xmax_sum, p.X);
+        b.learned = o.partition == MDR_PARTITION_LEARNED && c->mlp_ok && xmax_sum >= LEARNED_MIN_X;
         const i64 nwaves = (i64)c->ncu * c->quot_blocks_per_cu * 4;
         i64 tuned_chunks = 0;
         if (o.params == MDR_PARAMS_TUNED) {

@@ -44,6 +44,12 @@ struct FixupArgs {
 // Fixed per-slot cost (cycles) charged by the learned partition for a G
 // evaluation's set-up (frame, segment geometry, constant).
 constexpr long long SLOT_OVERHEAD_CYCLES = 1500;
+// The learned partition cuts chunks only for calls whose largest summed X is
+// at least this (the low end of the cost model's training range, 512 X in
+// [1e14, 1e16]): below it the model extrapolates, and its partition phase
+// (the network per cost piece, a second scan) costs about 0.35 ms against a
+// whole call of 0.4-3 ms; those calls take the static partition.
+constexpr long long LEARNED_MIN_X = 100000000000000LL;
 // Smallest automatic chunk (quotient positions): a chunk's set-up (locating its
 // slots, building their frames) costs about as much as a few thousand positions.
 constexpr long long MIN_CHUNK_POSITIONS = 1LL << 15;

@@ -66,7 +66,9 @@ enum {
     MDR_STRATEGY_FORCE_STREAMING = 3  /* compute_compressed: streaming for X > 1e5 */
 };
 /* chunk partition of the quotient phase: static = equal quotient positions per
- * chunk; learned = equal predicted cycles (the cost model); terms = equal
+ * chunk; learned = equal predicted cycles (the cost model; applied when the
+ * call's largest summed X is at least 1e14, the low end of the model's
+ * training range, and static otherwise); terms = equal
  * numbers of outer terms (G evaluations) per chunk, the reference GPU engine's
  * unit of work (one term per warp, mertens_cuda.cuh:307-403) */
 enum { MDR_PARTITION_STATIC = 0, MDR_PARTITION_LEARNED = 1, MDR_PARTITION_TERMS = 2 };
@@ -134,7 +136,8 @@ typedef struct {
     double table_bytes;                  /* bytes of the table arrays written */
     long long s1_comp_terms;             /* S1 positions answered by the compressed tier */
     int partition_used;                  /* MDR_PARTITION_* actually applied (learned falls
-                                            back to static without a cost model) */
+                                            back to static without a cost model and when
+                                            every summed X is below 1e14) */
     int graph_used;                      /* the call ran as one hipGraph launch */
     int graph_instantiated;              /* graphs instantiated (not updated in place) */
     double graph_ms;                     /* GPU time of the graph launches; the per-phase
