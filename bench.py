@@ -263,6 +263,59 @@ def config_times(expect):
     return rows
 
 
+def batch_times(E, ref, one_at_a_time=True):
+    """The 4096-X batch workload (log-uniform in [1e9, 1e14], seed 3): one call
+    per launch form (eager, graph, static partition), checked against the
+    compiled reference's values and each other, and 256 of the X one call
+    each (own table) for the batching gain."""
+    xs = load_json("xlists.json")["batch_4096_1e9_1e14_seed3"]
+    kw = dict(params=mdr_hip.PARAMS_TUNED, sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
+    bo = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, **kw)
+    bg = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, use_graph=1, **kw)
+    bs = mdr_hip.make_opts(partition=mdr_hip.PARTITION_STATIC, **kw)
+    E.batch(xs[:64], bo)
+    E.batch(xs[:64], bg)
+    t0 = time.perf_counter()
+    out = E.batch(xs, bo)
+    tb = time.perf_counter() - t0
+    bst = E.last_stats
+    t0 = time.perf_counter()
+    outg = E.batch(xs, bg)
+    tg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    outs = E.batch(xs, bs)
+    ts = time.perf_counter() - t0
+    checked = [i for i, x in enumerate(xs) if x in ref]
+    bad = sum(out[i] != ref[xs[i]] for i in checked) + sum(a != b for a, b in zip(out, outg))
+    bad += sum(a != b for a, b in zip(out, outs))
+    batch = {"workload": "4096 X log-uniform in [1e9, 1e14], seed 3", "n": len(xs), "seconds": round(tb, 4),
+             "evals_per_s": round(len(xs) / tb, 1),
+             "graph_seconds": round(tg, 4), "graph_evals_per_s": round(len(xs) / tg, 1),
+             "static_partition_seconds": round(ts, 4), "static_over_learned": round(ts / tb, 3),
+             "partition_used": "learned" if bst.partition_used == mdr_hip.PARTITION_LEARNED else
+                               "static (every X below 1e14, the cost model's training range)",
+             "checked_vs_reference": len(checked),
+             "table_U": bst.U, "table_ms": round(bst.table_ms, 2), "quotient_ms": round(bst.quotient_ms, 2)}
+    if one_at_a_time:
+        # the same X values, one independent call each (own table, eager and
+        # graph launches)
+        one = xs[:256]
+        so = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, **kw)
+        sg = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, use_graph=1, **kw)
+        t0 = time.perf_counter()
+        single = [E.mertens(x, so) for x in one]
+        t1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        single_g = [E.mertens(x, sg) for x in one]
+        t1g = time.perf_counter() - t0
+        bad += sum(single[i] != out[i] for i in range(len(one))) + sum(single_g[i] != out[i] for i in range(len(one)))
+        batch["one_at_a_time_evals_per_s"] = round(len(one) / t1, 1)
+        batch["one_at_a_time_graph_evals_per_s"] = round(len(one) / t1g, 1)
+        batch["speedup_vs_one_at_a_time"] = round((len(xs) / min(tb, tg)) / (len(one) / min(t1, t1g)), 2)
+    batch["mismatches"] = int(bad)
+    return batch
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +327,8 @@ def main():
     ap.add_argument("--no-static", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the M(1e12) / M(1e15) configuration timings")
     ap.add_argument("--no-large", action="store_true", help="skip the 64-X batch in [1e16, 1e18]")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 4096-X batch, --steps times (for kernel profiles of the batch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path would shard by "
@@ -281,6 +336,15 @@ def main():
         sys.exit(2)
     known = {int(k): v for k, v in load_json("known.json").items()}
     ref = {int(k): v for k, v in load_json("ref_values.json").items()}
+    if args.batch_only:
+        E = mdr_hip.Engine(0)
+        for i in range(args.steps):
+            b = batch_times(E, ref, one_at_a_time=False)
+            log("batch", i, b)
+            assert b["mismatches"] == 0
+        E.close()
+        print(json.dumps({"batch": b}), flush=True)
+        return
     expect_all = dict(ref)
     expect_all.update(known)
     X = args.x
@@ -356,51 +420,9 @@ def main():
 
     batch = None
     if not args.no_batch:
-        xs = load_json("xlists.json")["batch_4096_1e9_1e14_seed3"]
-        kw = dict(params=mdr_hip.PARAMS_TUNED, sched=mdr_hip.SCHED_DYNAMIC, no_table_reuse=1)
-        bo = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, **kw)
-        bg = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, use_graph=1, **kw)
-        bs = mdr_hip.make_opts(partition=mdr_hip.PARTITION_STATIC, **kw)
-        E.batch(xs[:64], bo)
-        E.batch(xs[:64], bg)
-        t0 = time.perf_counter()
-        out = E.batch(xs, bo)
-        tb = time.perf_counter() - t0
-        bst = E.last_stats
-        t0 = time.perf_counter()
-        outg = E.batch(xs, bg)
-        tg = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        outs = E.batch(xs, bs)
-        ts = time.perf_counter() - t0
-        checked = [i for i, x in enumerate(xs) if x in ref]
-        bad = sum(out[i] != ref[xs[i]] for i in checked) + sum(a != b for a, b in zip(out, outg))
-        bad += sum(a != b for a, b in zip(out, outs))
-        # one-at-a-time: the same X values, one independent call each (own table,
-        # eager and graph launches)
-        one = xs[:256]
-        so = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, **kw)
-        sg = mdr_hip.make_opts(partition=mdr_hip.PARTITION_LEARNED, use_graph=1, **kw)
-        t0 = time.perf_counter()
-        single = [E.mertens(x, so) for x in one]
-        t1 = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        single_g = [E.mertens(x, sg) for x in one]
-        t1g = time.perf_counter() - t0
-        bad += sum(single[i] != out[i] for i in range(len(one))) + sum(single_g[i] != out[i] for i in range(len(one)))
-        batch = {"workload": "4096 X log-uniform in [1e9, 1e14], seed 3", "n": len(xs), "seconds": round(tb, 4),
-                 "evals_per_s": round(len(xs) / tb, 1),
-                 "graph_seconds": round(tg, 4), "graph_evals_per_s": round(len(xs) / tg, 1),
-                 "static_partition_seconds": round(ts, 4), "static_over_learned": round(ts / tb, 3),
-                 "partition_used": "learned" if bst.partition_used == mdr_hip.PARTITION_LEARNED else
-                                   "static (every X below 1e14, the cost model's training range)",
-                 "one_at_a_time_evals_per_s": round(len(one) / t1, 1),
-                 "one_at_a_time_graph_evals_per_s": round(len(one) / t1g, 1),
-                 "speedup_vs_one_at_a_time": round((len(xs) / min(tb, tg)) / (len(one) / min(t1, t1g)), 2),
-                 "checked_vs_reference": len(checked), "mismatches": int(bad),
-                 "table_U": bst.U, "table_ms": round(bst.table_ms, 2), "quotient_ms": round(bst.quotient_ms, 2)}
+        batch = batch_times(E, ref)
         log("batch", batch)
-        assert bad == 0
+        assert batch["mismatches"] == 0
 
     large = None
     if not args.no_large:
