@@ -60,13 +60,16 @@ __device__ __forceinline__ void rcp_quad(double a, double b, double c, double d,
 // q = floor(y/n) for q <= 2^32 - 2 and 2n <= 2^32, from an estimate rn of 1/n
 // within 2^-47 relative.  2^52 + y*rn (one FMA) rounds the estimate, which is
 // within 2^-15 of y/n, to the nearest integer q0 in {q, q + 1} and leaves q0 in
-// the low mantissa word; the 32-bit remainder y - q0*n is in [0, n) for
-// q0 = q and in [2^32 - n, 2^32) for q0 = q + 1, so one comparison corrects.
+// the low mantissa word.  The true remainder r = y - q0*n lies in [0, n) for
+// q0 = q and in [-n, 0) for q0 = q + 1, so t = (q0*n - y - 1) mod 2^32 =
+// (-r - 1) mod 2^32 lies in [2^32 - n, 2^32) (at least n, as n <= 2^31) or in
+// [0, n): one multiply-add (q0*n + ~y, a single v_mad_u64_u32) and one
+// comparison correct the estimate.
 __device__ __forceinline__ uint32_t qdiv32n(uint32_t y_lo, double dy, uint32_t n, double rn) {
     const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
     const uint32_t q0 = (uint32_t)__double_as_longlong(qm);
-    const uint32_t r = y_lo - q0 * n;
-    return q0 - (r >= n ? 1u : 0u);
+    const uint32_t t = q0 * n + ~y_lo;
+    return q0 - (t < n ? 1u : 0u);
 }
 
 // Two Newton steps (set-up code outside the hot loops).

@@ -200,10 +200,11 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
     if (nmax < 2147483648ull) {
         // 2n <= 2^32, q <= U < 2^44.  q0 = nearest integer to the estimate
         // (2^52 + qd leaves it in the low mantissa bits) is q or q + 1, so the
-        // true remainder y - q0*n lies in [-n, n) and its low 32 bits decide
-        // the correction exactly; off = q - ud - 1 is the value's place in the
-        // compressed tier.  Two positions per step share one reciprocal.
-        const uint32_t ylo = (uint32_t)y;
+        // true remainder y - q0*n lies in [-n, n) and (q0*n - y - 1) mod 2^32,
+        // one multiply-add, is below n exactly when q0 = q + 1 (as in
+        // qdiv32n); off = q - ud - 1 is the value's place in the compressed
+        // tier.  Two positions per step share one reciprocal.
+        const uint32_t nylo = ~(uint32_t)y;
         const u64 base = (u64)tp.ud + 1;
         const uint32_t iters = lane_iters(cnt, lane);
         const uint32_t n0 = (uint32_t)nlo + (uint32_t)(ST * lane);
@@ -245,8 +246,8 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
             return run([&](uint32_t nn, double rn) -> int32_t {
                 const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
                 const u64 bits = (u64)__double_as_longlong(qm);
-                const uint32_t r = ylo - (uint32_t)bits * nn;
-                const uint32_t sub_lo = cK_lo + (r >= nn ? 1u : 0u);   // r >= nn: q0 = q + 1
+                const uint32_t tr = (uint32_t)bits * nn + nylo;
+                const uint32_t sub_lo = cK_lo + (tr < nn ? 1u : 0u);   // tr < nn: q0 = q + 1
                 const u64 off = bits - ((u64)cK_hi << 32 | sub_lo);
                 const uint32_t w = (uint32_t)(off >> 6);
                 const uint32_t line = __umulhi(w, 0xaaaaaaabu) >> 1;
@@ -260,8 +261,8 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
         return run([&](uint32_t nn, double rn) -> int32_t {
             const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
             const u64 q0 = (u64)__double_as_longlong(qm) & 0xfffffffffffffull;
-            const uint32_t r = ylo - (uint32_t)q0 * nn;
-            const bool over = r >= nn;                     // q0 = q + 1
+            const uint32_t tr = (uint32_t)q0 * nn + nylo;
+            const bool over = tr < nn;                     // q0 = q + 1
             const u64 off = q0 - base - (over ? 1ull : 0ull);
             const double offd = (qm - cbase) - (over ? 1.0 : 0.0);
             const uint32_t line = (uint32_t)__builtin_floor(
@@ -300,6 +301,17 @@ __device__ __forceinline__ uint32_t piece32(uint32_t q) {
     }
 }
 
+// mu * v of one position.  Piece 6 (v = q mod 2, so mu * v is -1, 0 or 1)
+// adds into an int32 partial sum: a lane of the 32-bit loop takes at most
+// 2^31 / 192 positions of a run (j < 2^31, stride 192), far below 2^31.  The
+// other pieces take one signed 32 x 32 -> 64-bit multiply-add.
+template <int P>
+__device__ __forceinline__ void s2_add(i64& s, int32_t& s6, uint32_t word, uint32_t sh, uint32_t q) {
+    const int mu = mu6_field(word, sh);
+    if constexpr (P == 6) s6 += (q & 1u) ? mu : 0;
+    else s += (i64)mu * (i64)(int32_t)piece32<P>(q);
+}
+
 template <int P>
 __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane) {
     const double dy = (double)y;
@@ -323,36 +335,37 @@ __device__ i64 s2_run(const TableParam& tp, u64 y, u64 t_lo, u64 t_hi, int lane)
         const uint32_t ylo = (uint32_t)y;
         const uint32_t t0 = (uint32_t)ts + (uint32_t)lane;
         const uint32_t* wp = tp.mu6 + (t0 >> 4);
+        int32_t s6 = 0;
         const uint32_t sh = 2u * (t0 & 15u);
         uint32_t j = 3u * t0 + 1u + (t0 & 1u);
-        uint32_t k = 0;
-        for (; k + 4 <= iters; k += 4) {
+        // the loop runs on j alone: j_end < 2^31 + 768 * 2^24 fits 32 bits
+        const uint32_t j_end = j + 768u * (iters >> 2);
+        for (; j != j_end;) {
             double r1, r2, r3, r4;
             rcp_quad((double)j, (double)(j + 192u), (double)(j + 384u), (double)(j + 576u), r1, r2, r3, r4);
             const uint32_t q1 = qdiv32n(ylo, dy, j, r1);
             const uint32_t q2 = qdiv32n(ylo, dy, j + 192u, r2);
             const uint32_t q3 = qdiv32n(ylo, dy, j + 384u, r3);
             const uint32_t q4 = qdiv32n(ylo, dy, j + 576u, r4);
-            s += (i64)mu6_field(wp[0], sh) * (i64)(int32_t)piece32<P>(q1);
-            s += (i64)mu6_field(wp[4], sh) * (i64)(int32_t)piece32<P>(q2);
-            s += (i64)mu6_field(wp[8], sh) * (i64)(int32_t)piece32<P>(q3);
-            s += (i64)mu6_field(wp[12], sh) * (i64)(int32_t)piece32<P>(q4);
+            s2_add<P>(s, s6, wp[0], sh, q1);
+            s2_add<P>(s, s6, wp[4], sh, q2);
+            s2_add<P>(s, s6, wp[8], sh, q3);
+            s2_add<P>(s, s6, wp[12], sh, q4);
             j += 768u;
             wp += 16;
         }
-        if (k + 2 <= iters) {
+        if (iters & 2u) {
             double r1, r2;
             rcp_pair((double)j, (double)(j + 192u), r1, r2);
             const uint32_t q1 = qdiv32n(ylo, dy, j, r1);
             const uint32_t q2 = qdiv32n(ylo, dy, j + 192u, r2);
-            s += (i64)mu6_field(wp[0], sh) * (i64)(int32_t)piece32<P>(q1);
-            s += (i64)mu6_field(wp[4], sh) * (i64)(int32_t)piece32<P>(q2);
+            s2_add<P>(s, s6, wp[0], sh, q1);
+            s2_add<P>(s, s6, wp[4], sh, q2);
             j += 384u;
             wp += 8;
-            k += 2;
         }
-        if (k < iters)
-            s += (i64)mu6_field(wp[0], sh) * (i64)(int32_t)piece32<P>(qdiv32n(ylo, dy, j, rcp_fast((double)j)));
+        if (iters & 1u) s2_add<P>(s, s6, wp[0], sh, qdiv32n(ylo, dy, j, rcp_fast((double)j)));
+        s += s6;
     }
     return s;
 }
