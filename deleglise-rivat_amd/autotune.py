@@ -31,7 +31,7 @@ sys.path.insert(0, HERE)
 import mdr_hip  # noqa: E402
 
 U_CAP = 390_000_000_000   # the library's cap on a 288 GB device (45 % of memory)
-LINE = 192
+LINE = 256
 
 
 def cutoff(X, fx, cx, cap=True):
@@ -108,11 +108,11 @@ class Tuner:
 
 def tune_decade(T, d, waves, quick):
     xs = [10**d] + ([int(round(10 ** (d + 0.5)))] if d < 18 else [])
-    cfg = {"fx": 0.7, "cx": 2.0 if d >= 13 else 0.95 if d >= 11 else 0.7, "ud": 1048512, "chunks": 0, "sieve": 0}
+    cfg = {"fx": 0.7, "cx": 2.0 if d >= 13 else 0.95 if d >= 11 else 0.7, "ud": 1048576, "chunks": 0, "sieve": 0}
     log(f"decade {d}: X = {xs}")
     fxs = [0.35, 0.5, 0.7, 1.0, 1.4, 2.0] if not quick else [0.5, 0.7, 1.0]
     cxs = [0.7, 1.0, 1.35, 2.0, 2.8, 4.0] if not quick else [1.35, 2.0, 2.8]
-    uds = [LINE * (2**k // LINE) for k in range(16, 24)] if not quick else [262080, 1048512, 4194240]
+    uds = [LINE * (2**k // LINE) for k in range(16, 24)] if not quick else [262144, 1048576, 4194304]
     chs = [0, waves * 4, waves * 8, waves * 16, waves * 32, waves * 64] if not quick else [0, waves * 32]
     T.sweep(xs, cfg, "fx", fxs)
     T.sweep(xs, cfg, "cx", cxs)

@@ -3,14 +3,17 @@
 // HBM layout of one table (all arrays device-resident, built by the sieve):
 //   Md    int16 [ud + 1]         M(n) for n <= ud (dense tier)
 //   cl    CLine [L]              compressed tier above ud: one 64-byte line per
-//                                192 values (three 64-value words); word w of a
-//                                line holds the nonzero and negative bit planes
-//                                of its values and M just before its first value
+//                                256 values, of which the 192 not divisible by 4
+//                                are stored (mu = 0 on multiples of 4) as three
+//                                64-value words; word w of a line holds the
+//                                nonzero and negative bit planes of its values
+//                                and M just before its first value
 //   mu6   u32   [ceil(T6/16)]    2-bit codes mu(j)+1 for j coprime to 6, indexed
 //                                by t = floor(j/3), so j(t) = 3t + 1 + (t & 1)
 // The reference keeps the same two tiers (mertens_dr.hpp:662-752) but stores
 // the planes in 256-value chunks and the checkpoints in a separate array; here
-// a scattered lookup touches exactly one 64-byte line.
+// a scattered lookup touches exactly one 64-byte line, which covers 256 values
+// in 0.25 B per value.
 #ifndef MDR_COMMON_HPP
 #define MDR_COMMON_HPP
 
@@ -22,7 +25,7 @@ typedef long long i64;
 typedef unsigned long long u64;
 
 // Sieve segment lengths are fixed per kernel instantiation (SIEVE_SHAPES in
-// mdr_kernels.hpp): multiples of 192 (compressed lines), 48 (packed mu words)
+// mdr_kernels.hpp): multiples of 256 (compressed lines), 48 (packed mu words)
 // and 16 (aligned 16-byte stencil loads).
 // Stencil period lcm(16, 4, 9, 25, 7, 11): weights of 2,3,5,7,11 and the square
 // flags of 4, 9 and 25 repeat with it; 16 | period keeps phases 16-byte aligned.
@@ -49,19 +52,52 @@ constexpr unsigned pattern3_byte(unsigned v) {   // v = n mod PATTERN3_PERIOD
 }
 // Values per compressed line; u_dense must be a multiple of it below U, and
 // every segment length is a multiple of it, so lines never straddle segments.
-constexpr int LINE_VALUES = 192;
+// A line therefore starts at n = 1 (mod 4): its multiples of 4 (mu = 0) sit
+// at the offsets r = 3 (mod 4) and are not stored, so its other LINE_KEPT
+// values, the kept values, fill three 64-bit words per plane.
+constexpr int LINE_VALUES = 256;
+constexpr int LINE_KEPT = 192;
 // Largest compressed tier (U - u_dense): line indices stay below 2^32.
 constexpr long long MAX_COMPRESSED_VALUES = (long long)LINE_VALUES * 4294967295LL;
+// Kept index of the last kept value at or below line offset r (0..255): of the
+// r + 1 values up to r, floor((r + 1) / 4) are multiples of 4.  A multiple of
+// 4 maps to the kept value before it, whose prefix sum it shares (mu = 0).
+constexpr uint32_t kept_index(uint32_t r) { return r - ((r + 1u) >> 2); }
+// The kept bits of 16 values from four nibble bytes (bits 0..2 of each byte;
+// bit 3, a multiple of 4, is zero and dropped): 12 bits, byte i at 3 i.
+constexpr uint32_t pack12(uint32_t v) {
+    uint32_t t = v & 0x07070707u;
+    t = (t | (t >> 5)) & 0x003f003fu;   // bytes 1 and 3 next to bytes 0 and 2
+    return (t | (t >> 10)) & 0xfffu;    // bits 16..21 next to bits 0..5
+}
+// The 48 kept bits of one 64-value word from its four nibble dwords (16
+// values each, byte i of dword j holding the values 16 j + 4 i .. + 3).
+constexpr u64 kept48(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
+    return (u64)(pack12(d0) | pack12(d1) << 12) | (u64)(pack12(d2) | pack12(d3) << 12) << 24;
+}
+// Inverse for one 64-value word: 48 kept bits to the 64 plane bits, zero at
+// the offsets 3 (mod 4).
+constexpr u64 expand48(u64 k) {
+    u64 x = (k & 0xfffull) | ((k & 0xfff000ull) << 4) | ((k & 0xfff000000ull) << 8) | ((k & 0xfff000000000ull) << 12);
+    x = (x & 0x003f003f003f003full) | ((x & 0x0fc00fc00fc00fc0ull) << 2);
+    return (x & 0x0707070707070707ull) | ((x & 0x3838383838383838ull) << 1);
+}
 
 // Sieve workgroup shapes (segment length, threads): the table they build is
 // the same; the autotuner picks one per decade of X.  Segment lengths are
-// 3 * 2^shift, multiples of 192 (lines), 48 (mu words) and 16.
+// 3 * 2^shift, multiples of 256 (lines), 48 (mu words) and 16.
 struct SieveShape {
     int seg, threads, shift;
 };
 constexpr SieveShape SIEVE_SHAPES[] = {{49152, 1024, 14}, {24576, 256, 13}, {98304, 1024, 15},
                                        {49152, 256, 14},  {49152, 512, 14},  {24576, 512, 13}};
 constexpr int N_SIEVE_SHAPES = (int)(sizeof(SIEVE_SHAPES) / sizeof(SIEVE_SHAPES[0]));
+constexpr bool sieve_shapes_fit() {
+    for (const SieveShape& sh : SIEVE_SHAPES)
+        if (sh.seg % LINE_VALUES || sh.seg % 48 || sh.seg != 3 << sh.shift) return false;
+    return true;
+}
+static_assert(sieve_shapes_fit(), "segments hold whole lines and mu words");
 constexpr int SEG_MAX = 98304;
 // Every segment base is a multiple of SIEVE_UNIT (the shortest segment); the
 // large-prime stage of the sieve writes a base as sb * SIEVE_UNIT with sb a
@@ -177,10 +213,13 @@ constexpr uint32_t mu_nibbles(uint32_t mu) {
 }
 
 struct alignas(64) CLine {
-    u64 pl[6];        // word w: pl[2w] = nonzero plane, pl[2w+1] = negative plane
-    int32_t ck[3];    // word w: M(value before its first value)
+    u64 pl[6];        // word w (kept values 64 w .. 64 w + 63): pl[2w] = nonzero
+                      // plane, pl[2w+1] = negative plane
+    int32_t ck[3];    // word w: M(value before its first kept value)
     int32_t pad;
 };
+static_assert(sizeof(CLine) == 64 && 3 * 64 == LINE_KEPT && LINE_VALUES - LINE_VALUES / 4 == LINE_KEPT,
+              "a line: three kept words and their checkpoints in 64 bytes");
 
 // Segment kinds of one G evaluation.
 enum SegType : uint8_t {

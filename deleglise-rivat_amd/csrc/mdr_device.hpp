@@ -162,7 +162,7 @@ __device__ __forceinline__ int mu6_mu(const uint32_t* mu6, i64 t) {
     return mu6_field(mu6[t >> 4], (uint32_t)(2 * (t & 15)));
 }
 
-// M from one compressed line: word sub (0..2), bit b (0..63) of the word.
+// M from one compressed line: word sub (0..2), kept bit b (0..63) of the word.
 // The prefix popcount over bits 0..b is the popcount of the word shifted left
 // by 63 - b (the bits above b leave the word), which needs no mask.
 __device__ __forceinline__ int32_t line_M(const CLine* L, uint32_t sub, uint32_t b) {
@@ -174,12 +174,29 @@ __device__ __forceinline__ int32_t line_M(const CLine* L, uint32_t sub, uint32_t
     return ck + (int)__popcll(nz) - 2 * (int)__popcll(ng);
 }
 
-// M(q) in the compressed tier (q > ud), off = q - ud - 1 < 192 * 2^32: the
-// value's line off / 192, word (off >> 6) mod 3 and bit off & 63.
+// M(q) in the compressed tier (q > ud), off = q - ud - 1 < 256 * 2^32: the
+// value's line off / 256 and, for the kept index k of its line offset
+// (kept_index), word k / 64 and bit k mod 64.
+static_assert(LINE_VALUES == 256, "a compressed value's line is off >> 8");
 __device__ __forceinline__ int32_t comp_M(const CLine* cl, u64 off) {
-    const u64 line = off / (u64)LINE_VALUES;
-    const uint32_t sub = (uint32_t)(off >> 6) - 3u * (uint32_t)line;
-    return line_M(cl + line, sub, (uint32_t)(off & 63));
+    const uint32_t k = kept_index((uint32_t)off & (LINE_VALUES - 1u));
+    return line_M(cl + (off >> 8), k >> 6, k & 63u);
+}
+
+// Plane bits of the 64 values at tier offsets 64 g .. 64 g + 63 (zero at the
+// multiples of 4): the group's 48 kept bits at kept offset 48 (g mod 4) of
+// line g / 4, expanded.
+__device__ __forceinline__ void raw_planes(const CLine* cl, u64 g, u64& nz, u64& ng) {
+    const CLine& L = cl[g >> 2];
+    const uint32_t o = 48u * (uint32_t)(g & 3);   // 0, 48, 96, 144
+    const uint32_t w = o >> 6, sh = o & 63u;      // the piece from bit sh of word w on
+    u64 kz = L.pl[2 * w] >> sh, kg = L.pl[2 * w + 1] >> sh;
+    if (sh > 16) {                                // fewer than 48 bits there: the next word
+        kz |= L.pl[2 * w + 2] << (64 - sh);
+        kg |= L.pl[2 * w + 3] << (64 - sh);
+    }
+    nz = expand48(kz & 0xffffffffffffull);
+    ng = expand48(kg & 0xffffffffffffull);
 }
 
 // M(q) from either tier (mertens_dr.hpp:707-712, :741-752).

@@ -322,7 +322,7 @@ void share_batch_cutoff(std::vector<XPlan>& plans, const mdr_opts& o) {
         bool inside = false;
         for (const XPlan& p : plans)
             if (p.kind == 2 && p.X <= ub) inside = true;
-        if (inside) ub = tune_u_cap() / 192 * 192;
+        if (inside) ub = tune_u_cap() / LINE_VALUES * LINE_VALUES;
         else window = tune_u_cap();
     }
     if (ub <= umax) return;
@@ -433,11 +433,11 @@ mdr_status prepare_table(mdr_ctx* c, TableStore& t, i64 U, i64 ud, i64 mu_lim, b
     if (U < 1) U = 1;
     if (ud > U) ud = U;
     if (ud < U && (ud % LINE_VALUES) != 0)
-        return fail(c, MDR_ERR_INVALID, "u_dense must be a multiple of 192 below U");
+        return fail(c, MDR_ERR_INVALID, "u_dense must be a multiple of 256 below U");
     if (mu_lim > U) return fail(c, MDR_ERR_INVALID, "mu_limit above U");
     if (mu_lim < 1) mu_lim = 1;
     if (U - ud > MAX_COMPRESSED_VALUES)
-        return fail(c, MDR_ERR_RANGE, "table above the compressed-tier limit (U - u_dense > 192 * (2^32 - 1))");
+        return fail(c, MDR_ERR_RANGE, "table above the compressed-tier limit (U - u_dense > 256 * (2^32 - 1))");
     if (U > SIEVE_MAX_U) return fail(c, MDR_ERR_RANGE, "table above the sieve's limit (U >= 24576 * 2^32)");
     if (shape < 0 || shape >= N_SIEVE_SHAPES) return fail(c, MDR_ERR_INVALID, "sieve shape out of range");
     const i64 seg = SIEVE_SHAPES[shape].seg;
@@ -798,7 +798,7 @@ mdr_status run_batch(mdr_ctx* c, std::vector<XPlan>& plans, const mdr_opts& o, l
     }
     // one dense tier for the batch: the largest any X asked for; lookups
     // alone keep a dense table only up to 5e7 (where |M| fits the int16 tier)
-    ud = any_sum ? std::min(U, ud) : U <= 50000000LL ? U : 1048512;
+    ud = any_sum ? std::min(U, ud) : U <= 50000000LL ? U : 1048576;
     if (ud < U) ud = (ud / LINE_VALUES) * LINE_VALUES;
     // sieve workgroup shape: explicit, else the tuned one of the largest X
     int shape = o.sieve_shape > 0 ? o.sieve_shape - 1 : 0;
@@ -1175,7 +1175,7 @@ mdr_status dirichlet_mertens(mdr_ctx* c, i64 X, long long* out, mdr_stats* stats
     L = std::max<i64>({L, r + 1, 100000});
     L = std::min<i64>({L, tune_u_cap(), X});
     L = std::max<i64>(L, std::min<i64>(X, r + 1));
-    const i64 ud = L <= 1048512 ? L : 1048512;
+    const i64 ud = L <= 1048576 ? L : 1048576;   // a whole number of lines
     mdr_status s = build_table(c, c->tab, L, ud, 1, stats);
     if (s != MDR_OK) return s;
     const TableParam tp = c->tab.tp();
@@ -1891,8 +1891,9 @@ __global__ void k_gather_M(TableParam tp, const i64* q, i64 n, int32_t* out) {
 // :741-752) for a dense limit ref_ud at or above this table's own (ud): one
 // thread per 64-value word w of chunk c0 + w / 4, covering n = ref_ud + 1 +
 // 64 w' ...; the word's bits are this table's plane bits at offset
-// n - ud - 1 (one or two of its words), its checkpoint M(n - 1).  Words past
-// lim read as zero with a zero checkpoint.
+// n - ud - 1 (one or two of its 64-value groups, expanded from their kept
+// bits), its checkpoint M(n - 1).  Words past lim read as zero with a zero
+// checkpoint.
 __global__ void k_export_chunks(TableParam tp, i64 lim, i64 ref_ud, i64 c0, i64 nwords, u64* chunks, int32_t* ck) {
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nwords) return;
@@ -1905,14 +1906,16 @@ __global__ void k_export_chunks(TableParam tp, i64 lim, i64 ref_ud, i64 c0, i64 
         const u64 d = (u64)(n0 - tp.ud - 1);   // >= 0: ref_ud >= ud
         const u64 gw = d >> 6;
         const uint32_t b = (uint32_t)(d & 63);
-        const CLine* L0 = tp.cl + gw / 3;
-        nz = L0->pl[2 * (gw % 3)] >> b;
-        ng = L0->pl[2 * (gw % 3) + 1] >> b;
-        // the next word holds values only when it starts at or below U
+        u64 z0, g0;
+        raw_planes(tp.cl, gw, z0, g0);
+        nz = z0 >> b;
+        ng = g0 >> b;
+        // the next group holds values only when it starts at or below U
         if (b && (i64)(64 * (gw + 1)) < tp.U - tp.ud) {
-            const CLine* L1 = tp.cl + (gw + 1) / 3;
-            nz |= L1->pl[2 * ((gw + 1) % 3)] << (64 - b);
-            ng |= L1->pl[2 * ((gw + 1) % 3) + 1] << (64 - b);
+            u64 z1, g1;
+            raw_planes(tp.cl, gw + 1, z1, g1);
+            nz |= z1 << (64 - b);
+            ng |= g1 << (64 - b);
         }
         const i64 cnt = lim - n0 + 1;
         if (cnt < 64) {

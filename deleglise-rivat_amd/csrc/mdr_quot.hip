@@ -210,7 +210,7 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
         const uint32_t n0 = (uint32_t)nlo + (uint32_t)(ST * lane);
         constexpr uint32_t D = 64u * ST;
         // int32 partial sums over at most 2^11 positions: the table holds
-        // q <= U < 192 * 2^32 < 2^40, where |M(q)| < 0.6 sqrt(q) < 2^19.3
+        // q <= U < 256 * 2^32 + 2^32, where |M(q)| < 0.6 sqrt(q) < 2^19.3
         // (|M(x)| / sqrt(x) stays below 0.6 through x = 10^16), so each
         // partial sum stays below 2^30.3
         auto run = [&](auto lookup) -> i64 {
@@ -235,40 +235,22 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
             }
             return acc;
         };
+        // The estimate's bit pattern is 0x433 << 52 | q0, so one 64-bit
+        // subtraction gives off = q0 - over - base (over folded into the low
+        // word of the subtrahend: base = ud + 1 = 1 mod 256, so that word is
+        // never 0xffffffff and cannot carry).  The line off >> 8 fits 32 bits
+        // (the tier holds fewer than 256 * 2^32 values), the kept index of
+        // off mod 256 gives word and bit.
         const u64 cK = 0x4330000000000000ull + base;
-        if (tp.L <= 0x55555555ll && (uint32_t)cK != 0xffffffffu) {
-            // fewer than 2^32 words (3 per line): the word index w = off >> 6
-            // fits 32 bits, so the line is w / 3 by a multiply-high.  The
-            // estimate's bit pattern is 0x433 << 52 | q0, so one 64-bit
-            // subtraction gives off = q0 - over - base (over folded into
-            // the low word of the subtrahend, which cannot carry).
-            const uint32_t cK_lo = (uint32_t)cK, cK_hi = (uint32_t)(cK >> 32);
-            return run([&](uint32_t nn, double rn) -> int32_t {
-                const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
-                const u64 bits = (u64)__double_as_longlong(qm);
-                const uint32_t tr = (uint32_t)bits * nn + nylo;
-                const uint32_t sub_lo = cK_lo + (tr < nn ? 1u : 0u);   // tr < nn: q0 = q + 1
-                const u64 off = bits - ((u64)cK_hi << 32 | sub_lo);
-                const uint32_t w = (uint32_t)(off >> 6);
-                const uint32_t line = __umulhi(w, 0xaaaaaaabu) >> 1;
-                return line_M(tp.cl + line, w - 3u * line, (uint32_t)off & 63u);
-            });
-        }
-        // more words: the line index off / 192 is taken in FP64: off is
-        // exact there (< 2^52) and (off + 1/2) / 192 is at least 1/384 away
-        // from an integer, far above the rounding error
-        const double cbase = 4503599627370496.0 + (double)base;   // 2^52 + base, exact
+        const uint32_t cK_lo = (uint32_t)cK, cK_hi = (uint32_t)(cK >> 32);
         return run([&](uint32_t nn, double rn) -> int32_t {
             const double qm = __builtin_fma(dy, rn, 4503599627370496.0);
-            const u64 q0 = (u64)__double_as_longlong(qm) & 0xfffffffffffffull;
-            const uint32_t tr = (uint32_t)q0 * nn + nylo;
-            const bool over = tr < nn;                     // q0 = q + 1
-            const u64 off = q0 - base - (over ? 1ull : 0ull);
-            const double offd = (qm - cbase) - (over ? 1.0 : 0.0);
-            const uint32_t line = (uint32_t)__builtin_floor(
-                __builtin_fma(offd, 1.0 / LINE_VALUES, 0.5 / LINE_VALUES));
-            const uint32_t sub = (uint32_t)(off >> 6) - 3u * line;
-            return line_M(tp.cl + line, sub, (uint32_t)(off & 63));
+            const u64 bits = (u64)__double_as_longlong(qm);
+            const uint32_t tr = (uint32_t)bits * nn + nylo;
+            const uint32_t sub_lo = cK_lo + (tr < nn ? 1u : 0u);   // tr < nn: q0 = q + 1
+            const u64 off = bits - ((u64)cK_hi << 32 | sub_lo);
+            const uint32_t k = kept_index((uint32_t)off & (LINE_VALUES - 1u));
+            return line_M(tp.cl + (uint32_t)(off >> 8), k >> 6, k & 63u);
         });
     }
     for (u64 i = lane; i < cnt; i += 64) {

@@ -37,6 +37,9 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
     __shared__ __align__(16) uint32_t nib32[SEGT / 16];   // per dword: nonzero nibble | negative nibble << 4
     __shared__ int32_t gsum[GROUPS];
     __shared__ int32_t gpre[GROUPS];
+    // kept planes of the segment's lines: per line 12 16-bit pieces of the
+    // nonzero plane (192 bits), then 12 of the negative plane
+    __shared__ __align__(8) uint16_t stg[SEGT / LINE_VALUES][2 * LINE_KEPT / 16];
     __shared__ uint32_t small_off[64];
     __shared__ uint32_t sq_off[64];
     uint8_t* acc = reinterpret_cast<uint8_t*>(acc32);
@@ -276,26 +279,25 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
     }
     __syncthreads();
 
-    // 6. one lane per 64-value word: bit planes from 16 nibble bytes, word sums
+    // 6. one lane per 64-value word: word sums from 16 nibble bytes and, for
+    //    words of a compressed line, the word's 48 kept bits per plane
+    //    (pack12 drops the multiples of 4: the word starts at n = 1 mod 4) as
+    //    three 16-bit pieces at kept offset 48 (word mod 4) of its line's
+    //    staging.  Words past U are zero, so a line's staging is whole.
+    const i64 gl0 = ((i64)base - a.ud) / LINE_VALUES;   // line of the segment's first value (< 0: dense)
     for (int k = tid; k < GROUPS; k += THREADS) {
         const uint4 b = reinterpret_cast<const uint4*>(nib32)[k];
-        auto lo16 = [](uint32_t v) {
-            v &= 0x0f0f0f0fu;
-            v = (v | (v >> 4)) & 0x00ff00ffu;
-            return (v | (v >> 8)) & 0xffffu;
-        };
-        const u64 nz = (u64)lo16(b.x) | ((u64)lo16(b.y) << 16) | ((u64)lo16(b.z) << 32) | ((u64)lo16(b.w) << 48);
-        const u64 ng = (u64)lo16(b.x >> 4) | ((u64)lo16(b.y >> 4) << 16) | ((u64)lo16(b.z >> 4) << 32) |
-                       ((u64)lo16(b.w >> 4) << 48);
+        const u64 nz = kept48(b.x, b.y, b.z, b.w), ng = kept48(b.x >> 4, b.y >> 4, b.z >> 4, b.w >> 4);
         gsum[k] = (int)__popcll(nz) - 2 * (int)__popcll(ng);
-        const u64 n0 = first + (u64)k * 64;
-        if (n0 > (u64)a.ud && n0 <= (u64)a.U) {
-            const u64 w = (n0 - (u64)a.ud - 1) >> 6;
-            const u64 line = w / 3u;
-            const uint32_t sub = (uint32_t)(w - 3u * line);
-            u64* pl = a.cl[line].pl + 2 * sub;
-            pl[0] = nz;
-            pl[1] = ng;
+        const int ll = k >> 2;
+        if (gl0 + ll >= 0 && gl0 + ll < a.L) {
+            uint16_t* pz = &stg[ll][3 * (k & 3)];
+            uint16_t* pn = pz + LINE_KEPT / 16;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                pz[i] = (uint16_t)(nz >> (16 * i));
+                pn[i] = (uint16_t)(ng >> (16 * i));
+            }
         }
     }
     __syncthreads();
@@ -318,15 +320,27 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
     }
     __syncthreads();
 
-    // 7. checkpoints of compressed words (segment-local; the fix-up adds the
-    //    segment offset) and dense prefix values
-    for (int k = tid; k < GROUPS; k += THREADS) {
-        const u64 n0 = first + (u64)k * 64;
-        if (n0 > (u64)a.ud && n0 <= (u64)a.U) {
-            const u64 w = (n0 - (u64)a.ud - 1) >> 6;
-            const u64 line = w / 3u;
-            a.cl[line].ck[(uint32_t)(w - 3u * line)] = gpre[k];
+    // 7. compressed lines, written whole by four lanes each: lane w < 3 the
+    //    planes of kept word w, lane 3 the checkpoints M(before the word) of
+    //    the three words, segment-local (the fix-up adds the segment offset):
+    //    the line's first value starts word 0, so its checkpoint is the
+    //    exclusive prefix of the line's first 64-value group, and each next
+    //    one adds the word sum of the kept word before it.
+    for (int t = tid; t < GROUPS; t += THREADS) {
+        const int ll = t >> 2, part = t & 3;
+        if (gl0 + ll < 0 || gl0 + ll >= a.L) continue;
+        const u64* sw = reinterpret_cast<const u64*>(stg[ll]);   // nonzero words 0..2, negative words 0..2
+        uint4 v;
+        if (part < 3) {
+            const u64 nz = sw[part], ng = sw[3 + part];
+            v = make_uint4((uint32_t)nz, (uint32_t)(nz >> 32), (uint32_t)ng, (uint32_t)(ng >> 32));
+        } else {
+            const int32_t c0 = gpre[4 * ll];
+            const int32_t c1 = c0 + (int)__popcll(sw[0]) - 2 * (int)__popcll(sw[3]);
+            const int32_t c2 = c1 + (int)__popcll(sw[1]) - 2 * (int)__popcll(sw[4]);
+            v = make_uint4((uint32_t)c0, (uint32_t)c1, (uint32_t)c2, 0u);
         }
+        reinterpret_cast<uint4*>(a.cl + (gl0 + ll))[part] = v;
     }
     if (first <= (u64)a.ud) {
         for (int i = tid; i < SEGT / 4; i += THREADS) {

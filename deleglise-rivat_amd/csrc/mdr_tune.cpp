@@ -37,7 +37,7 @@ struct Entry {
 std::once_flag g_once;
 std::vector<Entry> g_entries;
 double g_batch_factor = 1.0;
-// Largest tuned cutoff: 4e11 values (133 GB of compressed lines), lowered to
+// Largest tuned cutoff: 4e11 values (100 GB of compressed lines), lowered to
 // what the device's memory holds (tune_set_device_memory).
 constexpr long long U_CAP_MAX = 400000000000LL;
 constexpr long long WINDOW_U_MAX = 1000000000000LL;
@@ -99,7 +99,7 @@ TuneChoice tune_lookup(i64 X, bool streaming) {
     const int dec = (int)std::floor(std::log10((double)X));
     double fx = 0.7;
     double cx = X >= 10000000000000LL ? 2.0 : X >= 100000000000LL ? 0.95 : 0.70;
-    i64 ud = 1048512;
+    i64 ud = 1048576;
     for (const Entry& e : g_entries)
         if (e.decade == dec) {
             fx = e.fx;
@@ -122,10 +122,10 @@ TuneChoice tune_lookup(i64 X, bool streaming) {
         t.window = cap;
         u = std::min<i64>(u, WINDOW_U_MAX);
     }
-    u = ((u + 191) / 192) * 192;
+    u = ((u + LINE_VALUES - 1) / LINE_VALUES) * LINE_VALUES;
     t.u = u;
     t.cx = cx;
-    t.ud = std::min(u, (ud / 192) * 192);
+    t.ud = std::min(u, (ud / LINE_VALUES) * LINE_VALUES);
     return t;
 }
 
@@ -167,8 +167,8 @@ int tune_sieve_shape(i64 X) {
 i64 tune_u_cap() { return g_u_cap.load(); }
 
 void tune_set_device_memory(size_t bytes) {
-    // compressed lines take 1/3 byte per value: 45 % of the memory at most
-    const double cap = std::min((double)U_CAP_MAX, (double)bytes * 0.45 * 3.0);
+    // compressed lines take 1/4 byte per value: 45 % of the memory at most
+    const double cap = std::min((double)U_CAP_MAX, (double)bytes * 0.45 * 4.0);
     g_u_cap.store(std::max<long long>(20000000000LL, (long long)cap));
 }
 

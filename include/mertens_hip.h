@@ -92,7 +92,7 @@ typedef struct {
     int no_table_reuse;       /* 1: rebuild the table even when the cached one fits */
     /* explicit GPU split for MDR_PARAMS_TUNED (0 = from the tuning cache):
      * sieve cutoff (<= 1e12; a table whose compressed tier U - u_dense
-     * exceeds 192 * (2^32 - 1) values fails with MDR_ERR_RANGE), dense-tier
+     * exceeds 256 * (2^32 - 1) values fails with MDR_ERR_RANGE), dense-tier
      * limit and S1/S2 split factor */
     long long tune_u;
     long long tune_ud;
@@ -232,8 +232,8 @@ long long mdr_param_dense_cutoff(long long X, const mdr_opts* opts);
 long long mdr_param_streaming_cutoff(long long X, const mdr_opts* opts);
 
 /* Component entry points (parity tests).  mdr_table_build sieves n = 1..U
- * with the dense tier up to u_dense (a multiple of 192 when below U; U - u_dense
- * at most 192 * (2^32 - 1), else MDR_ERR_RANGE) and packed mu up to mu_limit; the reads
+ * with the dense tier up to u_dense (a multiple of 256 when below U; U - u_dense
+ * at most 256 * (2^32 - 1), else MDR_ERR_RANGE) and packed mu up to mu_limit; the reads
  * copy M(first..first+count-1) and mu(...) to host. */
 mdr_status mdr_table_build(mdr_ctx* ctx, long long U, long long u_dense, long long mu_limit);
 mdr_status mdr_table_read_M(mdr_ctx* ctx, long long first, long long count, int32_t* out);
@@ -243,7 +243,7 @@ mdr_status mdr_table_read_mu(mdr_ctx* ctx, long long first, long long count, int
  * living until mdr_table_free (which must come before mdr_shutdown of its
  * context).  A table holds M(n) for 0 <= n <= U and mu(n) for n <= mu_limit
  * (mu reads beyond it give 0, as SieveTable::get_mu does); u_dense is the
- * table's int16 tier (rounded down to a multiple of 192 below U; the values
+ * table's int16 tier (rounded down to a multiple of 256 below U; the values
  * read are the same for any u_dense).  Calls on a table are serialised with
  * the other calls on its context.
  *   create       U >= 0; 0 <= mu_limit <= U (0 = none); *out receives the table

@@ -3,9 +3,10 @@
 // first_multiple, mod_below_square and square_divides against plain
 // 64/128-bit remainders, for every prime from 256 up to the largest sieving
 // prime a table below SIEVE_MAX_U needs, at segment bases sb = 0, 1, 2^32 - 1
-// and seeded random ones; and the four-byte finalisation mu_swar and
-// mu_nibbles against the per-byte rule mu_byte.  Prints the number of checks
-// and "ok", or the first failure.
+// and seeded random ones; the four-byte finalisation mu_swar and mu_nibbles
+// against the per-byte rule mu_byte; and the compressed lines' kept-value
+// packing (kept_index, kept48, expand48) against per-bit rules.  Prints the
+// number of checks and "ok", or the first failure.
 #include "mdr_common.hpp"
 
 #include <cstdio>
@@ -117,6 +118,46 @@ int main(int argc, char** argv) {
         }
         ++checks;
         if (mu_nibbles(mu) != want) return fail("mu_nibbles", 0, mu, mu_nibbles(mu), want);
+    }
+    // compressed lines: kept_index against a count of the kept offsets; kept48
+    // of the nibble dwords of 64 seeded mu values (zero at the offsets 3 mod 4)
+    // against the kept bits in order, expand48 back to the plane bits
+    for (uint32_t r = 0, kept = 0; r < (uint32_t)LINE_VALUES; ++r) {
+        if (r % 4 != 3) ++kept;
+        ++checks;
+        if (kept_index(r) != kept - 1) return fail("kept_index", 0, r, kept_index(r), kept - 1);
+    }
+    for (int trial = 0; trial < 20000; ++trial) {
+        int8_t mu[64];
+        for (int r = 0; r < 64; ++r) {
+            const u64 x = rng() % 3;
+            mu[r] = (int8_t)(r % 4 == 3 ? 0 : (trial < 3 ? (int)trial - 1 : (int)x - 1));
+        }
+        uint32_t nib[4];
+        u64 nz = 0, ng = 0, knz = 0, kng = 0;
+        for (int j = 0, kb = 0; j < 4; ++j) {
+            nib[j] = 0;
+            for (int d = 0; d < 4; ++d) {
+                uint32_t bytes = 0;
+                for (int bb = 0; bb < 4; ++bb) bytes |= (uint32_t)(uint8_t)mu[16 * j + 4 * d + bb] << (8 * bb);
+                nib[j] |= mu_nibbles(bytes) << (8 * d);
+            }
+            for (int v = 16 * j; v < 16 * j + 16; ++v) {
+                nz |= (u64)(mu[v] != 0) << v;
+                ng |= (u64)(mu[v] < 0) << v;
+                if (v % 4 == 3) continue;
+                knz |= (u64)(mu[v] != 0) << kb;
+                kng |= (u64)(mu[v] < 0) << kb;
+                ++kb;
+            }
+        }
+        const u64 gz = kept48(nib[0], nib[1], nib[2], nib[3]);
+        const u64 gg = kept48(nib[0] >> 4, nib[1] >> 4, nib[2] >> 4, nib[3] >> 4);
+        checks += 4;
+        if (gz != knz) return fail("kept48 nonzero", 0, nz, gz, knz);
+        if (gg != kng) return fail("kept48 negative", 0, ng, gg, kng);
+        if (expand48(gz) != nz) return fail("expand48 nonzero", 0, gz, expand48(gz), nz);
+        if (expand48(gg) != ng) return fail("expand48 negative", 0, gg, expand48(gg), ng);
     }
     std::printf("%llu primes %llu checks ok\n", primes, checks);
     return 0;

@@ -140,7 +140,7 @@ int main() {
     // four contexts, held while the same thread computes M(X)
     {
         std::vector<std::unique_ptr<hip::DeviceTable>> held;
-        for (int k = 0; k < 6; ++k) held.push_back(std::make_unique<hip::DeviceTable>(100000 + 50000 * k, 192, 1000));
+        for (int k = 0; k < 6; ++k) held.push_back(std::make_unique<hip::DeviceTable>(100000 + 50000 * k, 256, 1000));
         check(DelégliseRivatEngine::compute_mertens(987654321LL) == 1697, "compute beside resident tables");
         for (int k = 0; k < 6; ++k) {
             const int64 q = 99999 + 50000 * k;

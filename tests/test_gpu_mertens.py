@@ -126,26 +126,24 @@ def test_dropin_windowed_cpp_program():
 
 def test_largest_table_and_external_1e18():
     """M(1e18) = -46758740 is not a reference value (OEIS A084237): a
-    cross-check at the largest X of the batch workloads, with the tuned cutoff
-    and with a 3.9e11-value table (U - u_dense above 2^38, so word indices
-    above 2^32 and line indices near 2^31 are read)."""
+    cross-check at the largest X of the batch workloads, with the tuned cutoff,
+    with a 3.9e11-value table (U - u_dense above 2^38) and with a table of
+    2^31 + 1 compressed lines (line indices past 2^31, read as unsigned 32-bit
+    integers)."""
     e = mdr_hip.Engine(0)
     try:
         X = 10**18
         assert e.mertens(X, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, no_table_reuse=1)) == -46758740
-        ud = 1048512
-        U = 390_000_000_000 // 192 * 192
+        ud = 1048576
+        U = 390_000_000_000 // 256 * 256
         assert U - ud > 2**38
         o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=U, tune_ud=ud, no_table_reuse=1)
         assert e.mertens(X, o) == -46758740
         assert e.last_stats.U == U
-        # the compressed lookups take the line index in 32-bit integers up to
-        # 3 * 0x55555555 words (word indices up to 2^32 - 2) and in FP64 above
-        for lines in (0x55555555, 0x55555556):
-            U = ud + 192 * lines
-            o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=U, tune_ud=ud, no_table_reuse=1)
-            assert e.mertens(X, o) == -46758740, lines
-            assert e.last_stats.U == U
+        U = ud + 256 * (2**31 + 1)
+        o = mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=U, tune_ud=ud, no_table_reuse=1)
+        assert e.mertens(X, o) == -46758740
+        assert e.last_stats.U == U
     finally:
         e.close()
 

@@ -30,8 +30,8 @@ def pre_ref(mu_ref):
 
 @pytest.fixture(scope="module")
 def table(engine):
-    # two tiers: dense up to 4032, compressed above
-    engine.table_build(LIM, 4032, LIM)
+    # two tiers: dense up to 4096, compressed above
+    engine.table_build(LIM, 4096, LIM)
     return engine
 
 

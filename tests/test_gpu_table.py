@@ -33,8 +33,8 @@ def cases():
     out = []
     for U in [1, 2, 3, 4, 63, 64, 65, 255, 256, 257, 13859, 13860, 13861, 17325, SEG - 1, SEG, SEG + 1,
               2 * SEG + 5, 277199, 277200, 277201, 1000033, LIM]:
-        for ud in sorted({U, 0, 192, 4032, SEG, SEG + 192, (U // 384) * 192}):
-            if ud > U or (ud < U and ud % 192):
+        for ud in sorted({U, 0, 256, 4096, SEG, SEG + 256, (U // 512) * 256}):
+            if ud > U or (ud < U and ud % 256):
                 continue
             for ml in sorted({U, max(1, U // 3)}):
                 out.append((U, ud, ml))
@@ -54,7 +54,7 @@ def test_table_matches_eratosthenes(engine, mu_ref, pre_ref, U, ud, ml):
 def test_large_table_windows(engine, known, ref_values):
     # A 2e10 table (two tiers, ~6 GB): mu windows against trial division,
     # M differences against mu, and M at the known / reference-computed points.
-    U, ud = 20_000_000_000, 128_000_064
+    U, ud = 20_000_000_000, 128_000_000
     print("large table", U, flush=True)
     engine.table_build(U, ud, 60_000_000)
     for X in [10**9, 10**10]:
@@ -85,13 +85,14 @@ def test_read_beyond_table_is_an_error(engine):
 
 
 def test_compressed_tier_limit_is_an_error(engine):
-    # the compressed tier holds at most 192 * (2^32 - 1) values (32-bit line
-    # indices); larger requests fail before anything is allocated
-    ud = 1048512
+    # the compressed tier holds at most 256 * (2^32 - 1) values (32-bit line
+    # indices); larger requests fail before anything is allocated, and an
+    # explicit cutoff above 1e12 is rejected as well
+    ud = 1048576
     with pytest.raises(mdr_hip.MdrError, match="RANGE|compressed-tier"):
-        engine.table_build(ud + 192 * 2**32, ud, 10**6)
-    with pytest.raises(mdr_hip.MdrError, match="RANGE|compressed-tier"):
-        engine.mertens(10**18, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=9 * 10**11,
+        engine.table_build(ud + 256 * 2**32, ud, 10**6)
+    with pytest.raises(mdr_hip.MdrError, match="INVALID|tune_u"):
+        engine.mertens(10**18, mdr_hip.make_opts(params=mdr_hip.PARAMS_TUNED, tune_u=10**12 + 256,
                                                  tune_ud=ud, no_table_reuse=1))
 
 
@@ -122,7 +123,7 @@ def test_segment_prefixes_across_a_large_table(engine, ref_values):
     # the offset each segment gets from the scan of the segment sums: M at
     # points spread over a 6e10 table against the compiled reference's values,
     # and M differences against trial division across segment edges
-    U, ud = 60_000_000_000, 1048512
+    U, ud = 60_000_000_000, 1048576
     engine.table_build(U, ud, 10**6)
     pts = [x for x in ref_values if x <= U][::9]
     assert len(pts) > 20

@@ -79,7 +79,7 @@ def test_table_object_matches_eratosthenes(engine, mu_ref, pre_ref, U, ud, ml):
 
 
 def test_table_objects_coexist_with_the_cached_table(engine, pre_ref, known):
-    a = mdr_hip.Table(engine, 1000033, 192 * 100, 1000033)
+    a = mdr_hip.Table(engine, 1000033, 256 * 100, 1000033)
     b = mdr_hip.Table(engine, 262147, 262147, 5000)
     try:
         assert engine.mertens(10**14, no_table_reuse=1) == known[10**14]   # rebuilds the context's table
@@ -94,7 +94,7 @@ def test_table_objects_coexist_with_the_cached_table(engine, pre_ref, known):
 def test_table_object_errors(engine):
     with pytest.raises(mdr_hip.MdrError):
         mdr_hip.Table(engine, 100, 0, 101)          # mu limit above U
-    with mdr_hip.Table(engine, 1000, 192, 1000) as t:
+    with mdr_hip.Table(engine, 1000, 256, 1000) as t:
         with pytest.raises(mdr_hip.MdrError):
             t.M(990, 20)                              # beyond U
         with pytest.raises(mdr_hip.MdrError):
@@ -107,7 +107,7 @@ def test_table_object_errors(engine):
         with pytest.raises(mdr_hip.MdrError):
             t.mu(2**63 - 5, 10)
         with pytest.raises(mdr_hip.MdrError):
-            t.chunks(192, 2**62, 1)
+            t.chunks(256, 2**62, 1)
 
 
 def test_dropin_tables_cpp_program():

@@ -66,7 +66,7 @@ def test_windowed_random_points_match_one_table(engine):
         X = int(10 ** rng.uniform(11, 15.5))
         cx = rng.choice([0.7, 1.35, 2.0, 2.8])
         u = int(X ** (2 / 3) * rng.uniform(0.3, 2.0))
-        o = dict(params=mdr_hip.PARAMS_TUNED, tune_u=u, tune_cx=cx, tune_ud=rng.choice([0, 65472, 1048512]))
+        o = dict(params=mdr_hip.PARAMS_TUNED, tune_u=u, tune_cx=cx, tune_ud=rng.choice([0, 65536, 1048576]))
         one = engine.mertens(X, opts(**o))
         w = rng.randrange(49152, max(49153, u // 2))
         print("windowed random", X, u, cx, w, flush=True)
@@ -80,13 +80,11 @@ def test_windowed_batch(engine, ref_values):
 
 
 def test_windowed_cutoff_beyond_the_one_table_limit(engine):
-    # u = 1e12 exceeds the resident compressed tier's 192 * (2^32 - 1) values;
-    # M(1e18) = -46758740 is the external value (OEIS A084237) the one-table
-    # path also reproduces
+    # u = 1e12 is 250 GB of compressed lines as one table; windows of 2e11
+    # hold 50 GB at a time.  M(1e18) = -46758740 is the external value (OEIS
+    # A084237) the one-table path also reproduces
     X = 10**18
     o = dict(params=mdr_hip.PARAMS_TUNED, tune_u=10**12, tune_cx=2.8)
-    with pytest.raises(mdr_hip.MdrError, match="RANGE|compressed-tier"):
-        engine.mertens(X, opts(**o))
     assert engine.mertens(X, opts(window=2 * 10**11, **o)) == -46758740
     st = engine.last_stats
     assert st.windows == 5 and st.U >= 10**12
