@@ -250,7 +250,11 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
             const uint32_t sub_lo = cK_lo + (tr < nn ? 1u : 0u);   // tr < nn: q0 = q + 1
             const u64 off = bits - ((u64)cK_hi << 32 | sub_lo);
             const uint32_t k = kept_index((uint32_t)off & (LINE_VALUES - 1u));
-            return line_M(tp.cl + (uint32_t)(off >> 8), k >> 6, k & 63u);
+            // line off >> 8 by one funnel shift of the two words (a plain
+            // 64-bit shift would be folded with the line size into a 64-bit
+            // shift and two masks)
+            const uint32_t line = __builtin_amdgcn_alignbit((uint32_t)(off >> 32), (uint32_t)off, 8u);
+            return line_M(tp.cl + line, k >> 6, k & 63u);
         });
     }
     for (u64 i = lane; i < cnt; i += 64) {
