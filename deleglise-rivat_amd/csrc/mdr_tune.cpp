@@ -142,8 +142,8 @@ i64 tune_batch_u(double xsum, double factor) {
     const TuneChoice t = tune_lookup((i64)xe, true);
     double u = (double)t.u * (factor > 0 ? factor : g_batch_factor);
     u = std::min(u, (double)WINDOW_U_MAX);
-    const i64 ui = ((i64)u / 192) * 192;
-    return ui > 0 ? ui : 192;
+    const i64 ui = ((i64)u / LINE_VALUES) * LINE_VALUES;
+    return ui > 0 ? ui : LINE_VALUES;
 }
 
 i64 tune_chunks(i64 X) {
