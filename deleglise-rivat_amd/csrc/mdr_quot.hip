@@ -267,23 +267,25 @@ __device__ i64 s1_comp(const TableParam& tp, u64 y, u64 nlo, u64 cnt, int lane) 
 
 // S2 summand of a quotient q < 2^32 in 32-bit modular arithmetic: every
 // value lies in [0, 2^32) (pieces other than 4) or in (-2^29, 2^29) (piece 4),
-// so the wrapped 32-bit result is exact.
+// so the wrapped 32-bit result is exact.  The floor forms of piece_eval are
+// folded with q - 2 floor(q/2) = q mod 2, t - 2 floor(t/2) = t mod 2 and
+// t - floor(t/2) = ceil(t/2) for t = floor(q/3), and floor(q/12) =
+// floor(t/4) (checked against the unfolded forms on the host).
 template <int P>
 __device__ __forceinline__ uint32_t piece32(uint32_t q) {
-    const uint32_t h = q >> 1, f = q >> 2;
-    if constexpr (P == 6) return q & 1u;
-    else if constexpr (P == 7 || P == 11) return q - h;
+    const uint32_t odd = q & 1u, f = q >> 2;
+    if constexpr (P == 6) return odd;
+    else if constexpr (P == 7 || P == 11) return q - (q >> 1);
     else if constexpr (P == 8 || P == 12) return q;
-    else if constexpr (P == 5) return q - 2u * h + f;
+    else if constexpr (P == 5) return odd + f;
     else {
         const uint32_t t = __umulhi(q, 0xaaaaaaabu) >> 1;   // q / 3
-        const uint32_t sx = t >> 1, w = t >> 2;              // q / 6, q / 12
-        if constexpr (P == 1) return q - 2u * h - t + 2u * sx + f - w;
-        else if constexpr (P == 2) return q - 2u * h - t + 2u * sx + f;
-        else if constexpr (P == 3) return q - 2u * h - t + f + sx;
-        else if constexpr (P == 4) return q - 2u * h - t + f;
-        else if constexpr (P == 9) return q - h - t + sx;
-        else return q - h - t;   // P == 10
+        if constexpr (P == 1) return odd - (t & 1u) + f - (t >> 2);
+        else if constexpr (P == 2) return odd - (t & 1u) + f;
+        else if constexpr (P == 3) return odd + f - (t - (t >> 1));
+        else if constexpr (P == 4) return odd + f - t;
+        else if constexpr (P == 9) return (q - (q >> 1)) - (t - (t >> 1));
+        else return (q - (q >> 1)) - t;   // P == 10
     }
 }
 
