@@ -49,7 +49,7 @@ ISSUE_PEAK_GINST = SIMDS * CLOCK_HZ / CYCLES_PER_INST / 1e9
 # VALU instructions per quotient position in the quotient kernel's inner loops
 # (gfx950 build of csrc/mdr_quot.hip, counted per loop iteration and divided by
 # the positions an iteration takes): dense S1 23 per 2 positions, compressed
-# S1 66 per 2, S2 48..88 per 4 depending on the piece (the cheapest is used,
+# S1 66 per 2, S2 48..80 per 4 depending on the piece (the cheapest is used,
 # so the issue time below is a lower bound).
 VALU_PER_S1_DENSE, VALU_PER_S1_COMP, VALU_PER_S2_MIN = 11.5, 33.0, 12.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_mertens_1e17.json")
