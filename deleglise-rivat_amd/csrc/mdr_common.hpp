@@ -212,6 +212,18 @@ constexpr uint32_t mu_nibbles(uint32_t mu) {
     return (c | (c >> 7) | (c >> 14) | (c >> 21)) & 0xffu;
 }
 
+// mu_nibbles(mu_swar(x, add)) without forming the mu bytes: a byte is
+// nonzero when it has no square flag, and negative (mu = -1) when its parity
+// bit equals [a & 0x7f > floor(log2 n)] (mu_byte: mu = s for a > thr, -s
+// otherwise, s = -1 for an odd parity).
+constexpr uint32_t mu_nibbles_direct(uint32_t x, uint32_t add) {
+    const uint32_t gt = ((x & 0x7f7f7f7fu) + add) & 0x80808080u;
+    const uint32_t nz = ~(x >> 7) & 0x01010101u;
+    const uint32_t ng = ~((x & 0x01010101u) ^ (gt >> 7)) & nz;
+    const uint32_t c = nz | (ng << 4);
+    return (c | (c >> 7) | (c >> 14) | (c >> 21)) & 0xffu;
+}
+
 struct alignas(64) CLine {
     u64 pl[6];        // word w (kept values 64 w .. 64 w + 63): pl[2w] = nonzero
                       // plane, pl[2w+1] = negative plane

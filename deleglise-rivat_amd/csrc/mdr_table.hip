@@ -245,11 +245,19 @@ __global__ __launch_bounds__(THREADS) void k_sieve_t(SieveArgs a) {
         const uint32_t gend = (uint32_t)len >> 4;               // groups wholly <= U
         const bool general = first < (1ull << 17);
         const uint32_t add_lo = (127u - tseg) * 0x01010101u, add_hi = (126u - tseg) * 0x01010101u;
+        // the mu bytes are read again only for the packed mu codes and the
+        // dense tier; elsewhere the SWAR groups go straight to their nibbles
+        const bool keep_mu = s < a.mu_segs || first <= (u64)a.ud;
         uint4* acc4 = reinterpret_cast<uint4*>(acc32);
         for (uint32_t g = tid; g < SEGT / 16; g += THREADS) {
             uint4 v = acc4[g];
             if (!general && g < gend && !(straddle && g == gb)) {
                 const uint32_t add = g < gb ? add_lo : add_hi;
+                if (!keep_mu) {
+                    nib32[g] = mu_nibbles_direct(v.x, add) | (mu_nibbles_direct(v.y, add) << 8) |
+                               (mu_nibbles_direct(v.z, add) << 16) | (mu_nibbles_direct(v.w, add) << 24);
+                    continue;
+                }
                 v.x = mu_swar(v.x, add);
                 v.y = mu_swar(v.y, add);
                 v.z = mu_swar(v.z, add);

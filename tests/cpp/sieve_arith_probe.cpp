@@ -4,7 +4,8 @@
 // 64/128-bit remainders, for every prime from 256 up to the largest sieving
 // prime a table below SIEVE_MAX_U needs, at segment bases sb = 0, 1, 2^32 - 1
 // and seeded random ones; the four-byte finalisation mu_swar and mu_nibbles
-// against the per-byte rule mu_byte; and the compressed lines' kept-value
+// against the per-byte rule mu_byte, and mu_nibbles_direct against
+// mu_nibbles of mu_swar; and the compressed lines' kept-value
 // packing (kept_index, kept48, expand48) against per-bit rules.  Prints the
 // number of checks and "ok", or the first failure.
 #include "mdr_common.hpp"
@@ -100,6 +101,9 @@ int main(int argc, char** argv) {
                 const uint32_t other = (uint32_t)rng();
                 const uint32_t x = (other & ~(0xffu << (8 * pos))) | (a << (8 * pos));
                 const uint32_t m = mu_swar(x, add);
+                ++checks;
+                if (mu_nibbles_direct(x, add) != mu_nibbles(m))
+                    return fail("mu_nibbles_direct", thr, x, mu_nibbles_direct(x, add), mu_nibbles(m));
                 for (int b = 0; b < 4; ++b) {
                     const uint8_t want = (uint8_t)mu_byte((uint8_t)(x >> (8 * b)), n);
                     ++checks;
