@@ -30,7 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 import mdr_hip  # noqa: E402
 
-U_CAP = 390_000_000_000   # the library's cap on a 288 GB device (45 % of memory)
+U_CAP = 500_000_000_000   # the library's one-table cap (45 % of a 288 GB device would hold 5.2e11)
 LINE = 256
 
 

@@ -37,9 +37,9 @@ struct Entry {
 std::once_flag g_once;
 std::vector<Entry> g_entries;
 double g_batch_factor = 1.0;
-// Largest tuned cutoff: 4e11 values (100 GB of compressed lines), lowered to
+// Largest tuned cutoff: 5e11 values (125 GB of compressed lines), lowered to
 // what the device's memory holds (tune_set_device_memory).
-constexpr long long U_CAP_MAX = 400000000000LL;
+constexpr long long U_CAP_MAX = 500000000000LL;
 constexpr long long WINDOW_U_MAX = 1000000000000LL;
 std::atomic<long long> g_u_cap{U_CAP_MAX};
 
