@@ -94,7 +94,7 @@ def test_large_mixed_batch_learned_equals_static(engine, xlists, ref_values):
     print("large batch, learned partition", flush=True)
     learned = engine.batch(xs, tuned(no_table_reuse=1, partition=mdr_hip.PARTITION_LEARNED))
     assert engine.last_stats.partition_used == mdr_hip.PARTITION_LEARNED
-    assert engine.last_stats.windows >= 2 and engine.last_stats.U > 4 * 10**11
+    assert engine.last_stats.windows >= 2 and engine.last_stats.U > 5 * 10**11   # above the one-table cap
     print("large batch, static partition", flush=True)
     static = engine.batch(xs, tuned(partition=mdr_hip.PARTITION_STATIC, sched=mdr_hip.SCHED_STATIC))
     assert engine.last_stats.partition_used == mdr_hip.PARTITION_STATIC
@@ -112,7 +112,7 @@ def test_batch_with_an_x_below_the_shared_cutoff_stays_one_table(engine, ref_val
     xs = [small, 10**18, 9 * 10**17 + 7]
     out = engine.batch(xs, tuned(no_table_reuse=1))
     st = engine.last_stats
-    assert st.windows == 0 and st.U <= 4 * 10**11
+    assert st.windows == 0 and st.U <= 5 * 10**11   # the one-table cap
     assert out[0] == ref_values[small]
     assert out[1] == -46758740
     assert out[2] == engine.mertens(xs[2], tuned(no_table_reuse=1))
