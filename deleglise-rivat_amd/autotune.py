@@ -117,9 +117,12 @@ def tune_decade(T, d, waves, quick):
     T.sweep(xs, cfg, "fx", fxs)
     T.sweep(xs, cfg, "cx", cxs)
     T.sweep(xs, cfg, "ud", uds)
-    # second pass on fx around the choice (the best cutoff moves with cx and the tier)
+    # second passes around the choices (the best cutoff moves with cx and the
+    # tier, and the cx grid is coarse)
     f = cfg["fx"]
     T.sweep(xs, cfg, "fx", sorted({round(f * m, 3) for m in (0.8, 1.0, 1.25)}))
+    c = cfg["cx"]
+    T.sweep(xs, cfg, "cx", sorted({round(min(4.9, c * m), 2) for m in (0.87, 1.0, 1.15)}))
     # sieve segment length / workgroup (0 = 49152/1024, 1 = 24576/256,
     # 2 = 98304/1024, 3 = 49152/256, 4 = 49152/512, 5 = 24576/512): the table is
     # the same, only its build time moves
